@@ -1,0 +1,67 @@
+"""
+Oracle: parameters, element data and units (test infrastructure only).
+
+Reads the JSON copies of the GFN1/GFN2 TOML parametrizations and the element
+data file shipped in ``dxtb_amd/data`` (plain data; no product code is
+imported).
+"""
+
+from __future__ import annotations
+
+import json
+import math
+from functools import lru_cache
+from pathlib import Path
+
+DATA = Path(__file__).resolve().parent.parent / "dxtb_amd" / "data"
+
+
+@lru_cache(maxsize=None)
+def load(name: str) -> dict:
+    with open(DATA / f"{name}.json") as fh:
+        return json.load(fh)
+
+
+def par(gfn: str) -> dict:
+    return load(gfn)
+
+
+ELEM = load("elements")
+SYMBOLS = ["X"] + ELEM["symbols"]
+Z = {s: i for i, s in enumerate(SYMBOLS)}
+
+BOHR = ELEM["bohr_angstrom"]
+AA2AU = 1.0 / BOHR
+EV2AU = 1.0 / ELEM["hartree_ev"]  # tad_mctc.units.EV2AU (xtb/base.py:33)
+KELVIN2AU = ELEM["kb_hartree_per_kelvin"]  # tad_mctc K2AU (scf/base.py:30)
+
+# D3 covalent radii: 4/3 * Pyykko 2009, in Bohr (tad_mctc radii.COV_D3)
+COV_D3 = [0.0] + [4.0 / 3.0 * r * AA2AU for r in ELEM["covalent_2009"]]
+# atomic radii of the H0 distance polynomial, in Bohr (xtb/base.py:138)
+ATOMIC_RADII = [0.0] + [r * AA2AU for r in ELEM["atomic"]]
+
+ANG = {"s": 0, "p": 1, "d": 2, "f": 3, "g": 4}
+
+
+def shell_lab(shell: str) -> tuple[int, int]:
+    """'2p' -> (n=2, l=1)."""
+    return int(shell[:-1]), ANG[shell[-1]]
+
+
+def elem(gfn: str, z: int) -> dict:
+    return par(gfn)["element"][SYMBOLS[z]]
+
+
+def valence(shells: list[str]) -> list[bool]:
+    """First shell of each angular momentum is valence; later duplicates are
+    not (param/module/utils.py:480-530)."""
+    seen = set()
+    out = []
+    for sh in shells:
+        l = shell_lab(sh)[1]
+        out.append(l not in seen)
+        seen.add(l)
+    return out
+
+
+PI = math.pi

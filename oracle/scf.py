@@ -1,0 +1,269 @@
+"""
+Oracle: SCF single point, energy and variational forces (test infrastructure only).
+
+Follows (paths relative to ``src/dxtb/_src``):
+
+* reference occupation and electron count -- ``scf/iterator.py:147-195``,
+  ``wavefunction/filling.py:180-259``
+* Fock build -- ``scf/pure/conversions.py:255-297``
+* generalized eigensolve (Cholesky, L^-1 F L^-T, eigh, back-transform) --
+  ``exlibs/xitorch/_impls/linalg/symeig.py:38-75`` via ``scf/pure/ovlp_diag.py:36-62``
+* Fermi smearing -- ``wavefunction/filling.py:428-742``: the occupations are
+  the Fermi-Dirac distribution whose chemical potential reproduces the
+  electron count of each spin channel; the oracle finds that root by
+  bisection to machine precision
+* density -- ``scf/utils.py:29-52``; charges and atomic multipoles --
+  ``scf/pure/conversions.py:203-252``
+* potentials -- ``components/interactions/base.py:135-181`` and the ES2/ES3/AES2
+  modules (see ``oracle.coulomb``)
+* fixed point -- ``scf/implicit/fixed_point.py:476-542``: root of
+  ``y - g(y)`` on the potential vector with xitorch ``broyden1``
+  (``exlibs/xitorch/_impls/optimize/root/rootsolver.py:42-261``,
+  ``_jacobian.py:51-120, 250-316``), alpha = -0.5, damp = 0.5, stop when
+  ``|dx| < x_tol`` and ``|y| < f_tol``
+* energies -- ``scf/base.py:510-619`` (equal partition of -TS) and
+  ``calculators/types/energy.py:336-337``
+* forces -- the reference differentiates the converged SCF with autograd
+  (``calculators/types/autograd.py:82-193``).  The oracle uses the
+  equivalent variational expression: at self-consistency the free energy is
+  stationary, so ``dE/dR = d/dR [E(R; P) - Tr(W S(R))]`` at fixed density P
+  and energy-weighted density W (the structure of the analytic GFN1
+  gradient, ``xtb/gfn1.py:311-313``).  Pinned against central finite
+  differences of the SCF energy.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import coulomb, hamiltonian, integrals, params
+from .basis import Basis, build
+
+
+class System:
+    def __init__(self, numbers, positions: torch.Tensor, gfn: str, charge: float = 0.0, etemp: float = 300.0):
+        self.numbers = [int(z) for z in numbers if int(z) != 0]
+        self.gfn = gfn
+        self.positions = positions
+        self.basis: Basis = build(self.numbers, gfn)
+        self.charge = float(charge)
+        self.kt = etemp * params.KELVIN2AU
+        b = self.basis
+        self.ao_shell = b.ao_shell()
+        self.ao_atom = b.ao_atom()
+        self.sh_atom = b.shell_atom()
+        self.nat = len(self.numbers)
+        self.nsh = b.nsh
+        self.nao = b.nao
+        ref = []
+        for sh in b.shells:
+            ref.append(params.elem(gfn, sh.z)["refocc"][sh.ush[1]])
+        refocc = torch.tensor(ref, dtype=torch.float64)
+        nsph = torch.tensor([2 * sh.l + 1 for sh in b.shells], dtype=torch.float64)
+        self.n0 = (refocc / nsph)[self.ao_shell]
+        nel = refocc.sum() - self.charge
+        # alpha/beta split for a closed/open shell without explicit spin
+        # (filling.py:180-259)
+        nint = torch.floor(nel + 0.5)
+        nuhf = torch.remainder(nint, 2)
+        diff = torch.minimum(nuhf, nel)
+        nb = (nel - diff) / 2.0
+        self.nab = torch.stack([nb + diff, nb])
+        self.is_gfn2 = gfn == "gfn2"
+
+    # ---- geometry-dependent parts (differentiable in positions) -----------
+    def build(self, positions: torch.Tensor):
+        b = self.basis
+        s, d, q = integrals.multipole_matrices(b, positions)
+        cn = hamiltonian.coordination_number(self.numbers, positions, self.gfn)
+        h0 = hamiltonian.h0_matrix(b, positions, s, cn)
+        gam = coulomb.es2_matrix(b, positions)
+        hd, shell_es3 = coulomb.es3_hubbard_derivs(b)
+        out = {"s": s, "h0": h0, "gam": gam, "hd": hd, "shell_es3": shell_es3}
+        if self.is_gfn2:
+            out["d"] = d
+            out["q"] = q
+            out["aes"] = coulomb.aes2_cache(b, positions)
+        out["erep"] = coulomb.repulsion_energy(self.numbers, positions, self.gfn)
+        return out
+
+    # ---- SCF maps ----------------------------------------------------------
+    def charges(self, m, p):
+        qorb = self.n0 - (p * m["s"]).sum(-1)
+        qdp = qqp = None
+        if self.is_gfn2:
+            # -sum_k P_ik D[:, k, i] over i of atom A
+            dorb = -(p.unsqueeze(0) * m["d"]).sum(-2).T  # (nao, 3)
+            qorb_q = -(p.unsqueeze(0) * m["q"]).sum(-2).T  # (nao, 6)
+            qdp = torch.zeros(self.nat, 3, dtype=p.dtype).index_add(0, self.ao_atom, dorb)
+            qqp = torch.zeros(self.nat, 6, dtype=p.dtype).index_add(0, self.ao_atom, qorb_q)
+        return qorb, qdp, qqp
+
+    def reduce_sh(self, qorb):
+        return torch.zeros(self.nsh, dtype=qorb.dtype).index_add(0, self.ao_shell, qorb)
+
+    def reduce_at(self, qsh):
+        return torch.zeros(self.nat, dtype=qsh.dtype).index_add(0, self.sh_atom, qsh)
+
+    def potential(self, m, qorb, qdp, qqp):
+        qsh = self.reduce_sh(qorb)
+        qat = self.reduce_at(qsh)
+        vsh = m["gam"] @ qsh
+        vat = torch.zeros(self.nat, dtype=qorb.dtype)
+        if m["shell_es3"]:
+            vsh = vsh + m["hd"] * qsh * qsh
+        else:
+            vat = vat + m["hd"] * qat * qat
+        vdp = vqp = None
+        if self.is_gfn2:
+            va, vdp, vqp = coulomb.aes2_potential(m["aes"], qat, qdp, qqp)
+            vat = vat + va
+        vsh = vsh + vat[self.sh_atom]
+        return vsh[self.ao_shell], vdp, vqp
+
+    def fock(self, m, vorb, vdp, vqp):
+        f = m["h0"] - 0.5 * m["s"] * (vorb.unsqueeze(-1) + vorb.unsqueeze(-2))
+        if self.is_gfn2:
+            for mat, v in ((m["d"], vdp), (m["q"], vqp)):
+                vo = v[self.ao_atom]  # (nao, k) ket-atom potential
+                tmp = 0.5 * torch.einsum("kij,jk->ij", mat, vo)
+                f = f - (tmp + tmp.T)
+        return f
+
+    def diagonalize(self, m, f):
+        l = torch.linalg.cholesky(m["s"])
+        linv = torch.linalg.inv(l)
+        fp = linv @ f @ linv.T
+        e, c = torch.linalg.eigh(0.5 * (fp + fp.T))
+        return e, linv.T @ c
+
+    def occupation(self, e):
+        """Fermi-Dirac occupations (2, nao) for the alpha/beta counts."""
+        kt = self.kt
+        occ = []
+        for n in self.nab:
+            lo, hi = e.min() - 1.0, e.max() + 1.0
+            for _ in range(200):
+                mu = 0.5 * (lo + hi)
+                nn = torch.special.expit(-(e - mu) / kt).sum()
+                if nn > n:
+                    hi = mu
+                else:
+                    lo = mu
+            mu = 0.5 * (lo + hi)
+            occ.append(torch.special.expit(-(e - mu) / kt))
+        return torch.stack(occ)
+
+    def solve_density(self, m, f):
+        e, c = self.diagonalize(m, f)
+        occ = self.occupation(e)
+        ftot = occ.sum(0)
+        p = (c * ftot) @ c.T
+        return p, e, c, occ
+
+    def pack(self, vorb, vdp, vqp):
+        if self.is_gfn2:
+            return torch.cat([vorb, vdp.reshape(-1), vqp.reshape(-1)])
+        return vorb
+
+    def unpack(self, x):
+        if self.is_gfn2:
+            n, a = self.nao, self.nat
+            return x[:n], x[n : n + 3 * a].view(a, 3), x[n + 3 * a :].view(a, 6)
+        return x, None, None
+
+    def g(self, m, x):
+        vorb, vdp, vqp = self.unpack(x)
+        p, *_ = self.solve_density(m, self.fock(m, vorb, vdp, vqp))
+        return self.pack(*self.potential(m, *self.charges(m, p)))
+
+    def electronic_energy(self, m, p, qorb, qdp, qqp):
+        e_h0 = (p * m["h0"]).sum()
+        qsh = self.reduce_sh(qorb)
+        qat = self.reduce_at(qsh)
+        e = e_h0 + 0.5 * qsh @ m["gam"] @ qsh
+        if m["shell_es3"]:
+            e = e + (m["hd"] * qsh**3 / 3.0).sum()
+        else:
+            e = e + (m["hd"] * qat**3 / 3.0).sum()
+        if self.is_gfn2:
+            e = e + coulomb.aes2_energy(m["aes"], qat, qdp, qqp).sum()
+        return e
+
+    def free_energy(self, occ):
+        eps = torch.finfo(torch.float64).eps
+        o = torch.clamp(occ, min=eps)
+        o1 = torch.clamp(1 - occ, min=eps)
+        return (torch.log(o**o * o1**o1)).sum() * self.kt
+
+
+def broyden1(fcn, x0, f_tol=1e-10, x_tol=1e-10, maxiter=300, alpha=-0.5, damp=0.5):
+    """xitorch broyden1 without line search (rootsolver.py:42-261) for the root of fcn."""
+    x = x0.clone()
+    y = fcn(x)
+    ynorm = y.norm()
+    if ynorm <= f_tol * torch.finfo(y.dtype).eps:
+        return x, 0, True
+    cns, dns = [], []
+    g0 = -alpha
+    xprev, yprev = x, y
+
+    def mv(v):
+        r = g0 * v
+        for c, d in zip(cns, dns):
+            r = r + c * torch.dot(d, v)
+        return r
+
+    def rmv(v):
+        r = g0 * v
+        for c, d in zip(cns, dns):
+            r = r + d * torch.dot(c, v)
+        return r
+
+    for it in range(maxiter):
+        dx = -mv(y)
+        dxn = dx.norm()
+        xnew = x + damp * dx
+        ynew = fcn(xnew)
+        ynorm_new = ynew.norm()
+        ddx, ddy = xnew - xprev, ynew - yprev
+        v = rmv(ddx)
+        cns.append(ddx - mv(ddy))
+        dns.append(v / torch.dot(ddy, v))
+        xprev, yprev = xnew, ynew
+        if dxn < x_tol and ynorm_new < f_tol:
+            return xnew, it + 1, True
+        x, y = xnew, ynew
+    return x, maxiter, False
+
+
+class Result(dict):
+    pass
+
+
+def singlepoint(numbers, positions, gfn, charge=0.0, tol=1e-10, maxiter=300, forces=False):
+    """Total energy (repulsion + electronic + -TS) and optionally forces."""
+    sysm = System(numbers, positions, gfn, charge)
+    with torch.no_grad():
+        m = sysm.build(positions)
+        n = sysm.nao + (9 * sysm.nat if sysm.is_gfn2 else 0)
+        x0 = torch.zeros(n, dtype=torch.float64)
+        xs, niter, conv = broyden1(lambda x: x - sysm.g(m, x), x0, f_tol=tol, x_tol=tol, maxiter=maxiter)
+        vorb, vdp, vqp = sysm.unpack(xs)
+        f = sysm.fock(m, vorb, vdp, vqp)
+        p, e, c, occ = sysm.solve_density(m, f)
+        q = sysm.charges(m, p)
+        eel = sysm.electronic_energy(m, p, *q)
+        ts = sysm.free_energy(occ)
+        etot = m["erep"].sum() + eel + ts
+    res = Result(energy=etot, eel=eel, ts=ts, erep=m["erep"].sum(), iterations=niter, converged=conv,
+                 density=p, emo=e, occupation=occ, charges=q, potential=(vorb, vdp, vqp), coeffs=c)
+    if forces:
+        w = (c * (occ.sum(0) * e)) @ c.T
+        pos = positions.detach().clone().requires_grad_(True)
+        m2 = sysm.build(pos)
+        qq = sysm.charges(m2, p)
+        lag = m2["erep"].sum() + sysm.electronic_energy(m2, p, *qq) - (w * m2["s"]).sum()
+        (g,) = torch.autograd.grad(lag, pos)
+        res["forces"] = -g
+    return res
