@@ -1,0 +1,117 @@
+"""
+ctypes binding of the HIP library (C ABI of ``include/dxtb_hip.h``).
+
+The library is loaded from the package directory.  If it is missing the
+import of any compute function fails loudly: there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int, c_void_p
+from pathlib import Path
+
+import torch
+
+LIBPATH = Path(__file__).resolve().parent / "libdxtb_hip.so"
+
+DXTB_EINVAL = 1001
+DXTB_EUNSUP = 1002
+DXTB_ENOCONV = 1003
+
+_ERRORS = {
+    DXTB_EINVAL: "invalid argument",
+    DXTB_EUNSUP: "unsupported option",
+    DXTB_ENOCONV: "iterative solve did not converge",
+}
+
+
+class DxtbBasis(ctypes.Structure):
+    _fields_ = [
+        ("nbatch", c_int),
+        ("nat_max", c_int),
+        ("nsh_max", c_int),
+        ("nao_max", c_int),
+        ("sh_atom", c_void_p),
+        ("sh_ush", c_void_p),
+        ("sh_ao", c_void_p),
+        ("nush", c_int),
+        ("ush_l", c_void_p),
+        ("ush_np", c_void_p),
+        ("ush_alpha", c_void_p),
+        ("ush_coeff", c_void_p),
+        ("item_mol", c_void_p),
+        ("item_pair", c_void_p),
+        ("class_off", c_int * 10),
+    ]
+
+
+# name -> argtypes; every entry point returns int
+_SIGNATURES: dict[str, list] = {
+    "dxtb_multipole_fwd": [POINTER(DxtbBasis), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "dxtb_multipole_vjp": [POINTER(DxtbBasis), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+}
+
+_lib = None
+
+
+def exported_names() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def load() -> ctypes.CDLL:
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIBPATH.exists():
+            raise RuntimeError(
+                f"HIP library {LIBPATH} not found: build it with `python -m dxtb_amd.build` "
+                "(there is no CPU fallback)."
+            )
+        lib = ctypes.CDLL(str(LIBPATH))
+        for name, args in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = c_int
+        _lib = lib
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = _ERRORS.get(rc, f"HIP error {rc}")
+        raise RuntimeError(f"{what} failed: {msg} (code {rc})")
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensor passed to the HIP library must be contiguous")
+    return t.data_ptr()
+
+
+def stream_ptr(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def basis_struct(bb) -> DxtbBasis:
+    d = bb.dev
+    s = DxtbBasis()
+    s.nbatch = bb.nbatch
+    s.nat_max = bb.nat_max
+    s.nsh_max = bb.nsh_max
+    s.nao_max = bb.nao_max
+    s.sh_atom = ptr(d["sh_atom"])
+    s.sh_ush = ptr(d["sh_ush"])
+    s.sh_ao = ptr(d["sh_ao"])
+    s.nush = int(d["ush_l"].numel())
+    s.ush_l = ptr(d["ush_l"])
+    s.ush_np = ptr(d["ush_np"])
+    s.ush_alpha = ptr(d["ush_alpha"])
+    s.ush_coeff = ptr(d["ush_coeff"])
+    s.item_mol = ptr(d["item_mol"])
+    s.item_pair = ptr(d["item_pair"])
+    for i in range(10):
+        s.class_off[i] = int(bb.class_off[i])
+    return s

@@ -1,0 +1,227 @@
+"""
+Basis sets and the device-resident description of a batch of molecules.
+
+* Contracted Gaussians: STO-nG expansion of every parametrized Slater shell
+  (``src/dxtb/_src/basis/slater.py:69-145``), the GFN1 H-2s Gram-Schmidt step
+  (``basis/ortho.py:74-110``) and the exact unit-self-overlap rescaling of
+  the pair builder (``integral/driver/pytorch/impls/pairs.py:181-204``).
+  Built once per parametrization for all elements, in numpy (FP64).
+* AO order (``basis/indexhelper.py``): atoms in input order, shells in
+  parameter order, 2l+1 spherical functions per shell, p as (y, z, x).
+* ``BatchBasis``: padded per-molecule shell maps plus the ordered shell-pair
+  work lists of the integral kernels, grouped by (l_bra, l_ket), on device.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import param
+
+MAXP = 8  # primitives per shell in the device tables
+_DFACT = (1.0, 1.0, 3.0, 15.0, 105.0)
+_ITYPE = (0, 4, 7, 9, 10)
+
+
+def _self_overlap(alpha: np.ndarray, coeff: np.ndarray, l: int) -> float:
+    p = alpha[:, None] + alpha[None, :]
+    pref = (math.pi / p) ** 1.5 * _DFACT[l] / (2.0 * p) ** l
+    return float((coeff[:, None] * coeff[None, :] * pref).sum())
+
+
+def slater_to_gauss(ng: int, n: int, l: int, zeta: float) -> tuple[np.ndarray, np.ndarray]:
+    """Normalized STO-nG expansion of one Slater shell (basis/slater.py:69-145)."""
+    if not 1 <= ng <= 6 or n <= l or zeta <= 0.0:
+        raise ValueError(f"invalid STO-nG request ng={ng} n={n} l={l} zeta={zeta}")
+    itype = 15 + l if (n == 6 and ng == 6) else n + _ITYPE[l] - 1
+    tab = param.stong_table(ng)
+    coeff = tab[0, itype].copy()
+    alpha = tab[1, itype] * zeta * zeta
+    coeff *= (2.0 / math.pi * alpha) ** 0.75 * np.sqrt(4.0 * alpha) ** l / math.sqrt(_DFACT[l])
+    return alpha, coeff / math.sqrt(_self_overlap(alpha, coeff, l))
+
+
+def _s_overlap(ai, aj, ci, cj) -> float:
+    p = ai[:, None] + aj[None, :]
+    return float((np.sqrt(math.pi / p) ** 3 * ci[:, None] * cj[None, :]).sum())
+
+
+def orthogonalize(a1, c1, a2, c2) -> tuple[np.ndarray, np.ndarray]:
+    """Second s function orthonormalized against the first (basis/ortho.py:74-110)."""
+    ov = _s_overlap(a1, a2, c1, c2)
+    an = np.concatenate([a2, a1])
+    cn = np.concatenate([c2, -ov * c1])
+    return an, cn / math.sqrt(_s_overlap(an, an, cn, cn))
+
+
+class ShellTable:
+    """All contracted shells of one parametrization (unique shells)."""
+
+    def __init__(self, gfn: str) -> None:
+        par = param.get(gfn)
+        self.gfn = gfn
+        self.first = np.full(param.MAXZ + 1, -1, dtype=np.int64)  # first unique shell of Z
+        self.count = np.zeros(param.MAXZ + 1, dtype=np.int64)
+        ls, nps, al, co, zs = [], [], [], [], []
+        eps = np.finfo(np.float64).eps
+        for z in range(1, param.MAXZ + 1):
+            if not par.has(z):
+                continue
+            e = par.element(z)
+            val = par.valence(z)
+            self.first[z] = len(ls)
+            self.count[z] = len(e["shells"])
+            prev = None
+            for i, (n, l) in enumerate(par.shells(z)):
+                a, c = slater_to_gauss(int(e["ngauss"][i]), n, l, float(e["slater"][i]))
+                if gfn == "gfn1" and not val[i]:
+                    a, c = orthogonalize(prev[0], prev[1], a, c)
+                prev = (a, c)
+                # exact unit self overlap (impls/pairs.py:181-204)
+                s = _self_overlap(a, c, l)
+                cnorm = c / (math.sqrt(max(s, eps)) + eps)
+                if len(a) > MAXP:
+                    raise ValueError("too many primitives")
+                ls.append(l)
+                nps.append(len(a))
+                al.append(np.pad(a, (0, MAXP - len(a))))
+                co.append(np.pad(cnorm, (0, MAXP - len(a))))
+                zs.append(z)
+        self.l = np.array(ls, dtype=np.int32)
+        self.np = np.array(nps, dtype=np.int32)
+        self.alpha = np.array(al, dtype=np.float64)
+        self.coeff = np.array(co, dtype=np.float64)
+        self.z = np.array(zs, dtype=np.int32)
+        self._dev: dict = {}
+
+    def device(self, dev: torch.device) -> dict:
+        key = str(dev)
+        if key not in self._dev:
+            self._dev[key] = {
+                "l": torch.as_tensor(self.l, device=dev),
+                "np": torch.as_tensor(self.np, device=dev),
+                "alpha": torch.as_tensor(self.alpha, device=dev).contiguous(),
+                "coeff": torch.as_tensor(self.coeff, device=dev).contiguous(),
+            }
+        return self._dev[key]
+
+
+_TABLES: dict = {}
+
+
+def shell_table(gfn: str) -> ShellTable:
+    if gfn not in _TABLES:
+        _TABLES[gfn] = ShellTable(gfn)
+    return _TABLES[gfn]
+
+
+@dataclass
+class BatchBasis:
+    """Padded shell/AO maps of a batch and its shell-pair work lists."""
+
+    gfn: str
+    numbers: np.ndarray  # (B, nat_max) int, 0 = padding
+    nat: np.ndarray  # (B,)
+    nsh: np.ndarray
+    nao: np.ndarray
+    nat_max: int
+    nsh_max: int
+    nao_max: int
+    sh_atom: np.ndarray  # (B, nsh_max) int32
+    sh_ush: np.ndarray
+    sh_ao: np.ndarray
+    sh_l: np.ndarray
+    ao_atom: np.ndarray  # (B, nao_max) int32 (0 for padding)
+    ao_shell: np.ndarray
+    item_mol: np.ndarray
+    item_pair: np.ndarray
+    class_off: np.ndarray  # (10,)
+    dev: dict
+
+    @property
+    def nbatch(self) -> int:
+        return int(self.numbers.shape[0])
+
+
+def build_batch(numbers, gfn: str, device: torch.device | str) -> BatchBasis:
+    """Shell maps and ordered shell-pair lists for a (padded) batch."""
+    nums = np.atleast_2d(np.asarray(numbers.cpu() if torch.is_tensor(numbers) else numbers, dtype=np.int64))
+    table = shell_table(gfn)
+    B, nat_max = nums.shape
+    nat = (nums != 0).sum(1)
+    cnt = table.count[nums] * (nums != 0)
+    if ((nums != 0) & (table.first[nums] < 0)).any():
+        bad = sorted(set(nums[(nums != 0) & (table.first[nums] < 0)].tolist()))
+        raise ValueError(f"Elements {bad} are not parametrized in {gfn}.")
+    nsh = cnt.sum(1)
+    nsh_max = int(max(nsh.max(), 1))
+    if nsh_max > 0xFFFF:
+        raise ValueError("too many shells per molecule")
+    sh_atom = np.zeros((B, nsh_max), dtype=np.int32)
+    sh_ush = np.zeros((B, nsh_max), dtype=np.int32)
+    sh_l = np.zeros((B, nsh_max), dtype=np.int32)
+    for b in range(B):
+        # atoms of molecule b, each repeated by its shell count
+        za = nums[b, : nat[b]]
+        reps = table.count[za]
+        atoms = np.repeat(np.arange(nat[b]), reps)
+        first = np.repeat(table.first[za], reps)
+        within = np.arange(len(atoms)) - np.repeat(np.cumsum(reps) - reps, reps)
+        ush = first + within
+        sh_atom[b, : len(atoms)] = atoms
+        sh_ush[b, : len(atoms)] = ush
+        sh_l[b, : len(atoms)] = table.l[ush]
+    valid = np.arange(nsh_max)[None, :] < nsh[:, None]
+    nsph = np.where(valid, 2 * sh_l + 1, 0)
+    sh_ao = (np.cumsum(nsph, 1) - nsph).astype(np.int32)
+    nao = nsph.sum(1)
+    nao_max = int(max(nao.max(), 1))
+    ao_atom = np.zeros((B, nao_max), dtype=np.int32)
+    ao_shell = np.zeros((B, nao_max), dtype=np.int32)
+    for b in range(B):
+        reps = nsph[b, : nsh[b]]
+        ao_shell[b, : nao[b]] = np.repeat(np.arange(nsh[b]), reps)
+        ao_atom[b, : nao[b]] = sh_atom[b, ao_shell[b, : nao[b]]]
+
+    # ordered shell pairs (A, B) of every molecule, grouped by class 3*lA + lB
+    mols, pairs, cls = [], [], []
+    for b in range(B):
+        n = int(nsh[b])
+        a_idx, b_idx = np.divmod(np.arange(n * n), n)
+        mols.append(np.full(n * n, b, dtype=np.int32))
+        pairs.append(((a_idx << 16) | b_idx).astype(np.int32))
+        cls.append(3 * sh_l[b, a_idx] + sh_l[b, b_idx])
+    mols = np.concatenate(mols)
+    pairs = np.concatenate(pairs)
+    cls = np.concatenate(cls)
+    order = np.argsort(cls, kind="stable")
+    item_mol = mols[order]
+    item_pair = pairs[order]
+    counts = np.bincount(cls, minlength=9)
+    class_off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+
+    dev = torch.device(device)
+    td = table.device(dev)
+    t = lambda x: torch.as_tensor(np.ascontiguousarray(x), device=dev)  # noqa: E731
+    d = {
+        "sh_atom": t(sh_atom),
+        "sh_ush": t(sh_ush),
+        "sh_ao": t(sh_ao),
+        "item_mol": t(item_mol),
+        "item_pair": t(item_pair),
+        "ush_l": td["l"],
+        "ush_np": td["np"],
+        "ush_alpha": td["alpha"],
+        "ush_coeff": td["coeff"],
+        "ao_atom": t(ao_atom).long(),
+        "ao_shell": t(ao_shell).long(),
+        "sh_atom_l": t(sh_atom).long(),
+    }
+    return BatchBasis(
+        gfn, nums, nat, nsh, nao, int(nat_max), nsh_max, nao_max, sh_atom, sh_ush, sh_ao, sh_l,
+        ao_atom, ao_shell, item_mol, item_pair, class_off, d,
+    )

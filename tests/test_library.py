@@ -1,0 +1,45 @@
+"""The C-ABI library builds, loads and exports every symbol of include/dxtb_hip.h (CPU only)."""
+
+from __future__ import annotations
+
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from dxtb_amd import _lib
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared() -> set[str]:
+    text = "\n".join(p.read_text() for p in (ROOT / "include").glob("*.h"))
+    return set(re.findall(r"^\s*int\s+(dxtb_\w+)\s*\(", text, flags=re.M))
+
+
+def test_library_present():
+    assert _lib.LIBPATH.exists(), "run `python -m dxtb_amd.build` first"
+
+
+def test_exports_match_header():
+    declared = _declared()
+    assert declared, "no entry points declared"
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIBPATH)], capture_output=True, text=True)
+    exported = set(re.findall(r"\bT (dxtb_\w+)", out.stdout))
+    assert declared <= exported, f"missing exports: {sorted(declared - exported)}"
+    assert declared == set(_lib.exported_names()), "ctypes signatures out of sync with the header"
+
+
+def test_ctypes_load_and_bind():
+    lib = _lib.load()
+    for name in _lib.exported_names():
+        assert getattr(lib, name).restype is not None
+
+
+def test_invalid_arguments_rejected_without_gpu():
+    """Argument checks run on the host before any launch."""
+    lib = _lib.load()
+    st = _lib.DxtbBasis()  # all zero: invalid
+    assert lib.dxtb_multipole_fwd(st, None, None, None, None, None, None, None) == _lib.DXTB_EINVAL
+    assert lib.dxtb_multipole_vjp(st, None, None, None, None, None, None) == _lib.DXTB_EINVAL
