@@ -46,10 +46,42 @@ class DxtbBasis(ctypes.Structure):
     ]
 
 
+class DxtbParams(ctypes.Structure):
+    _fields_ = [
+        ("gfn", c_int),
+        ("es3_shell", c_int),
+        ("es2_average", c_int),
+        ("rep_light", c_int),
+        ("enscale", c_double),
+        ("es2_gexp", c_double),
+        ("rep_kexp", c_double),
+        ("rep_klight", c_double),
+        ("aes_dmp3", c_double),
+        ("aes_dmp5", c_double),
+        ("aes_kexp", c_double),
+        ("aes_shift", c_double),
+        ("aes_rmax", c_double),
+    ] + [
+        (name, c_void_p)
+        for name in (
+            "ush_lev", "ush_kcn", "ush_shpoly", "ush_hub", "ush_hd3", "ush_valence", "hscale", "z_rad",
+            "z_en", "z_rcov", "kpair", "z_gam3", "z_arep", "z_zeff", "z_mprad", "z_mpvcn", "z_dkernel",
+            "z_qkernel",
+        )
+    ]
+
+
 # name -> argtypes; every entry point returns int
 _SIGNATURES: dict[str, list] = {
     "dxtb_multipole_fwd": [POINTER(DxtbBasis), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "dxtb_multipole_vjp": [POINTER(DxtbBasis), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "dxtb_setup": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 9 + [c_void_p],
+    "dxtb_fock": [POINTER(DxtbBasis)] + [c_void_p] * 12 + [c_double, c_void_p, c_void_p],
+    "dxtb_charges": [POINTER(DxtbBasis)] + [c_void_p] * 17,
+    "dxtb_potential": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 14 + [c_void_p],
+    "dxtb_fermi": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p],
+    "dxtb_forces_multipole": [POINTER(DxtbBasis)] + [c_void_p] * 9,
+    "dxtb_forces_classical": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 19,
 }
 
 _lib = None

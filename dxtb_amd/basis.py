@@ -204,6 +204,10 @@ def build_batch(numbers, gfn: str, device: torch.device | str) -> BatchBasis:
     counts = np.bincount(cls, minlength=9)
     class_off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
 
+    # first shell and shell count of every atom
+    at_nsh = (table.count[nums] * (nums != 0)).astype(np.int32)
+    at_sh0 = (np.cumsum(at_nsh, 1) - at_nsh).astype(np.int32)
+
     dev = torch.device(device)
     td = table.device(dev)
     t = lambda x: torch.as_tensor(np.ascontiguousarray(x), device=dev)  # noqa: E731
@@ -220,6 +224,14 @@ def build_batch(numbers, gfn: str, device: torch.device | str) -> BatchBasis:
         "ao_atom": t(ao_atom).long(),
         "ao_shell": t(ao_shell).long(),
         "sh_atom_l": t(sh_atom).long(),
+        "ao_atom_i": t(ao_atom),
+        "ao_shell_i": t(ao_shell),
+        "at_sh0": t(at_sh0),
+        "at_nsh": t(at_nsh),
+        "numbers_i": t(nums.astype(np.int32)),
+        "nat": t(nat.astype(np.int32)),
+        "nsh": t(nsh.astype(np.int32)),
+        "nao": t(nao.astype(np.int32)),
     }
     return BatchBasis(
         gfn, nums, nat, nsh, nao, int(nat_max), nsh_max, nao_max, sh_atom, sh_ush, sh_ao, sh_l,

@@ -1,0 +1,167 @@
+"""
+Calculator facade of the HIP path: ``GFN1Calculator`` / ``GFN2Calculator``
+with ``get_energy`` and ``get_forces``.
+
+Mirrors the reference calculator API (``src/dxtb/_src/calculators/gfn1.py:38-70``,
+``gfn2.py:38-69``, ``types/abc.py:88-170``, ``types/base.py:453-884``):
+
+* ``Calculator(numbers, opts=None, device=None, dtype=torch.float64)`` with
+  integer ``numbers`` of shape (nat,) or zero-padded (B, nat);
+* ``get_energy(positions, chrg=0, spin=None)`` -> total energy, summed over
+  atoms: (,) or (B,);
+* ``get_forces(positions, chrg=0, spin=None, grad_mode="autograd")`` -> -dE/dR;
+  like the reference, it requires ``positions.requires_grad`` (RuntimeError
+  otherwise); the forces come from the hand-written backward kernels;
+* results are cached by tensor identity of the inputs; ``reset()`` clears
+  the cache (``types/base.py:758-763``).
+
+Dispersion (D3/D4) is not part of this path: the energy is the reference's
+total energy with ``exclude=["disp"]`` (``types/base.py:612-663``).
+"""
+
+from __future__ import annotations
+
+import warnings
+
+import numpy as np
+import torch
+
+from . import scf as _scf
+from .basis import build_batch
+
+
+class DtypeError(ValueError):
+    pass
+
+
+class DeviceError(ValueError):
+    pass
+
+
+class SCFConvergenceWarning(UserWarning):
+    pass
+
+
+class SCFConvergenceError(RuntimeError):
+    pass
+
+
+def _opts_to_scf(opts: dict | None) -> _scf.SCFOptions:
+    o = _scf.SCFOptions()
+    if not opts:
+        return o
+    for key, val in opts.items():
+        if key in ("f_atol", "f_tol"):
+            o.f_tol = float(val)
+        elif key in ("x_atol", "x_tol"):
+            o.x_tol = float(val)
+        elif key == "maxiter":
+            o.maxiter = int(val)
+        elif key in ("fermi_etemp", "etemp"):
+            o.etemp = float(val)
+        elif key == "damp":
+            o.damp = float(val)
+        elif key in ("verbosity", "scf_mode", "scp_mode", "mixer", "int_driver", "batch_mode",
+                     "force_convergence", "exclude"):
+            continue
+        else:
+            raise ValueError(f"Unknown option '{key}'.")
+    return o
+
+
+class _Calculator:
+    gfn = ""
+
+    def __init__(self, numbers: torch.Tensor, *, opts: dict | None = None, device=None,
+                 dtype: torch.dtype = torch.float64, **_):
+        if numbers.dtype not in (torch.int16, torch.int32, torch.int64, torch.long):
+            raise DtypeError(f"Atomic numbers must be an integer tensor, got {numbers.dtype}.")
+        if dtype != torch.float64:
+            raise DtypeError("The MI355X path computes in float64 only.")
+        self.device = torch.device(device) if device is not None else numbers.device
+        if self.device.type != "cuda":
+            raise DeviceError("The MI355X path runs on a GPU device (cuda:N); there is no CPU fallback.")
+        self.dtype = dtype
+        self.numbers = numbers.to(self.device)
+        self.batched = numbers.ndim == 2
+        nums = np.atleast_2d(numbers.detach().cpu().numpy())
+        self.opts = dict(opts or {})
+        self.scf_opts = _opts_to_scf(opts)
+        self.force_convergence = bool(self.opts.get("force_convergence", False))
+        self.bb = build_batch(nums, self.gfn, self.device)
+        self.cache: dict = {}
+        self._ncalcs = 0
+
+    def reset(self) -> None:
+        self.cache.clear()
+
+    # ------------------------------------------------------------------
+    def _check(self, positions):
+        if positions.dtype != self.dtype:
+            raise DtypeError(f"positions have dtype {positions.dtype}, expected {self.dtype}")
+        if positions.device != self.device:
+            raise DeviceError(f"positions are on {positions.device}, expected {self.device}")
+
+    def _charges(self, chrg):
+        B = self.bb.nbatch
+        if isinstance(chrg, (int, float)):
+            if self.batched and chrg != 0:
+                raise ValueError("Cannot set charge for batched calculations with a single float or integer. "
+                                 "Please provide a 1D tensor of charges.")
+            return np.full(B, float(chrg))
+        return np.asarray(chrg.detach().cpu() if torch.is_tensor(chrg) else chrg, dtype=np.float64).reshape(B)
+
+    def _key(self, positions, chrg, spin):
+        cid = id(chrg) if torch.is_tensor(chrg) else chrg
+        return (id(positions), positions._version, cid, spin)
+
+    def singlepoint(self, positions, chrg=0, spin=None):
+        self._check(positions)
+        key = self._key(positions, chrg, spin)
+        if key in self.cache:
+            return self.cache[key]
+        if spin not in (None, 0):
+            raise NotImplementedError("explicit spin is not supported on this path")
+        pos = positions.detach().reshape(self.bb.nbatch, self.bb.nat_max, 3)
+        state = _scf.GeometryState(self.bb, pos, self._charges(chrg), self.scf_opts)
+        res = _scf.run_scf(state)
+        if not res.converged.all():
+            msg = f"SCF did not converge for {int((~res.converged).sum())} molecule(s)."
+            if self.force_convergence:
+                raise SCFConvergenceError(msg)
+            warnings.warn(msg, SCFConvergenceWarning, stacklevel=3)
+        self.cache[key] = res
+        self._ncalcs += 1
+        return res
+
+    def get_energy(self, positions, chrg=0, spin=None, **_):
+        e = self.singlepoint(positions, chrg, spin).energy
+        return e if self.batched else e[0]
+
+    def get_forces(self, positions, chrg=0, spin=None, grad_mode="autograd", **_):
+        if grad_mode not in ("autograd", "backward", "functorch", "row"):
+            raise ValueError(f"Unknown grad_mode '{grad_mode}'.")
+        if not positions.requires_grad:
+            raise RuntimeError("Position tensor needs `requires_grad=True` to compute forces.")
+        res = self.singlepoint(positions, chrg, spin)
+        f = _scf.forces(res)
+        return f.reshape(positions.shape)
+
+    def get_property(self, name, positions, chrg=0, spin=None, **kw):
+        if name == "energy":
+            return self.get_energy(positions, chrg, spin)
+        if name == "forces":
+            return self.get_forces(positions, chrg, spin, **kw)
+        raise NotImplementedError(f"Property '{name}' is not implemented on the MI355X path.")
+
+
+class GFN1Calculator(_Calculator):
+    """GFN1-xTB on the MI355X path (``calculators/gfn1.py:38-70``)."""
+
+    gfn = "gfn1"
+
+
+class GFN2Calculator(_Calculator):
+    """GFN2-xTB on the MI355X path (``calculators/gfn2.py:38-69``)."""
+
+    gfn = "gfn2"

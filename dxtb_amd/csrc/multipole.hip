@@ -176,8 +176,6 @@ __global__ __launch_bounds__(256) void k_multipole_fwd(dxtb_basis_t bs, int off,
         const double w = kab * ew;
 #pragma unroll
         for (int q = 0; q < NCB; ++q) {
-          constexpr int dummy = 0;
-          (void)dummy;
           const int bx = cart_exp(LB, q, 0), by = cart_exp(LB, q, 1), bz = cart_exp(LB, q, 2);
           const double x0 = rx[bx], y0 = ry[by], z0 = rz[bz];
           const double wyz = w * y0 * z0;
@@ -249,12 +247,26 @@ __global__ __launch_bounds__(256) void k_multipole_fwd(dxtb_basis_t bs, int off,
 // Derivative of a bra primitive w.r.t. its centre:
 //   d/dA_x (x-A_x)^i e^{-a(x-A_x)^2} = 2a (x-A_x)^{i+1} e - i (x-A_x)^{i-1} e.
 // The ket-centred moments depend on A - B only, so d/dB = -d/dA.
-template <int LA, int LB, bool MP>
+// Adjoint sources: either explicit adjoint matrices (adjS/adjD/adjQ) or, in
+// SCF mode, the variational adjoints of the converged free energy built on
+// the fly from the density P and the energy-weighted density W:
+//   adjS_mn = P_mn (f_AB - v_n) - W_mn,  adjD_mn = -P_mn vd(atom n),
+//   adjQ_mn = -P_mn vq(atom n)   (f_AB: H0 shell-pair factor, v: potentials)
+struct AdjSrc {
+  const double* adjS;
+  const double* adjD;
+  const double* adjQ;
+  const double* P;
+  const double* W;
+  const double* fsh;
+  const double* vsh;
+  const double* vdp;
+  const double* vqp;
+};
+
+template <int LA, int LB, bool MP, bool SCF>
 __global__ __launch_bounds__(256) void k_multipole_vjp(dxtb_basis_t bs, int off, int nitems,
-                                                       const double* __restrict__ pos,
-                                                       const double* __restrict__ adjS,
-                                                       const double* __restrict__ adjD,
-                                                       const double* __restrict__ adjQ,
+                                                       const double* __restrict__ pos, AdjSrc src,
                                                        double* __restrict__ grad) {
   constexpr int NA = 2 * LA + 1, NB = 2 * LB + 1, NCB = Cart<LB>::n;
   constexpr int NI = LA + 2, NJ = LB + (MP ? 3 : 1);
@@ -276,28 +288,57 @@ __global__ __launch_bounds__(256) void k_multipole_vjp(dxtb_basis_t bs, int off,
   for (int q = 0; q < NCB; ++q)
 #pragma unroll
     for (int t = 0; t < NC; ++t) adj[q][t] = 0.0;
+  double fab = 0.0, vb = 0.0;
+  const double* vdb = nullptr;
+  const double* vqb = nullptr;
+  if constexpr (SCF) {
+    fab = src.fsh[((size_t)c.b * bs.nsh_max + c.ia) * bs.nsh_max + c.ib];
+    vb = src.vsh[(size_t)c.b * bs.nsh_max + c.ib];
+    if constexpr (MP) {
+      vdb = src.vdp + ((size_t)c.b * bs.nat_max + c.atB) * 3;
+      vqb = src.vqp + ((size_t)c.b * bs.nat_max + c.atB) * 6;
+    }
+  }
 #pragma unroll
   for (int mb = 0; mb < NB; ++mb) {
     const size_t e0 = (size_t)c.b * nn + (size_t)c.row * n + c.col0 + mb;
     double a[NC];
-    a[0] = adjS[e0];
+    double pe = 0.0;
+    if constexpr (SCF) {
+      pe = src.P[e0];
+      a[0] = pe * (fab - vb) - src.W[e0];
+    } else {
+      a[0] = src.adjS[e0];
+    }
     if constexpr (MP) {
       const size_t db = (size_t)c.b * 3 * nn + (size_t)c.row * n + c.col0 + mb;
       const size_t qb = (size_t)c.b * 6 * nn + (size_t)c.row * n + c.col0 + mb;
       double t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0, t5 = 0;
       a[1] = a[2] = a[3] = 0.0;
-      if (adjD != nullptr) {
-        a[1] = adjD[db];
-        a[2] = adjD[db + nn];
-        a[3] = adjD[db + 2 * nn];
-      }
-      if (adjQ != nullptr) {
-        t0 = adjQ[qb];
-        t1 = adjQ[qb + nn];
-        t2 = adjQ[qb + 2 * nn];
-        t3 = adjQ[qb + 3 * nn];
-        t4 = adjQ[qb + 4 * nn];
-        t5 = adjQ[qb + 5 * nn];
+      if constexpr (SCF) {
+        a[1] = -pe * vdb[0];
+        a[2] = -pe * vdb[1];
+        a[3] = -pe * vdb[2];
+        t0 = -pe * vqb[0];
+        t1 = -pe * vqb[1];
+        t2 = -pe * vqb[2];
+        t3 = -pe * vqb[3];
+        t4 = -pe * vqb[4];
+        t5 = -pe * vqb[5];
+      } else {
+        if (src.adjD != nullptr) {
+          a[1] = src.adjD[db];
+          a[2] = src.adjD[db + nn];
+          a[3] = src.adjD[db + 2 * nn];
+        }
+        if (src.adjQ != nullptr) {
+          t0 = src.adjQ[qb];
+          t1 = src.adjQ[qb + nn];
+          t2 = src.adjQ[qb + 2 * nn];
+          t3 = src.adjQ[qb + 3 * nn];
+          t4 = src.adjQ[qb + 4 * nn];
+          t5 = src.adjQ[qb + 5 * nn];
+        }
       }
       // traceless transform is linear: adjoint onto raw xx, xy, xz, yy, yz, zz
       a[4] = t0 - 0.5 * t2 - 0.5 * t5;
@@ -401,21 +442,45 @@ int launch_fwd(const dxtb_basis_t& bs, const double* pos, const double* fsh, dou
 }
 
 template <int LA, int LB>
-int launch_vjp(const dxtb_basis_t& bs, const double* pos, const double* aS, const double* aD,
-               const double* aQ, double* grad, hipStream_t st) {
+int launch_vjp(const dxtb_basis_t& bs, const double* pos, const AdjSrc& src, bool mp, bool scf, double* grad,
+               hipStream_t st) {
   const int c = 3 * LA + LB;
   const int off = bs.class_off[c], cnt = bs.class_off[c + 1] - off;
   if (cnt <= 0) return 0;
   const long threads = (long)cnt * (2 * LA + 1);
   const int blk = 256;
   const dim3 grid((unsigned)((threads + blk - 1) / blk));
-  if (aD != nullptr || aQ != nullptr)
-    hipLaunchKernelGGL((k_multipole_vjp<LA, LB, true>), grid, dim3(blk), 0, st, bs, off, cnt, pos, aS, aD,
-                       aQ, grad);
-  else
-    hipLaunchKernelGGL((k_multipole_vjp<LA, LB, false>), grid, dim3(blk), 0, st, bs, off, cnt, pos, aS,
-                       aD, aQ, grad);
+  if (scf) {
+    if (mp)
+      hipLaunchKernelGGL((k_multipole_vjp<LA, LB, true, true>), grid, dim3(blk), 0, st, bs, off, cnt, pos, src,
+                         grad);
+    else
+      hipLaunchKernelGGL((k_multipole_vjp<LA, LB, false, true>), grid, dim3(blk), 0, st, bs, off, cnt, pos, src,
+                         grad);
+  } else {
+    if (mp)
+      hipLaunchKernelGGL((k_multipole_vjp<LA, LB, true, false>), grid, dim3(blk), 0, st, bs, off, cnt, pos, src,
+                         grad);
+    else
+      hipLaunchKernelGGL((k_multipole_vjp<LA, LB, false, false>), grid, dim3(blk), 0, st, bs, off, cnt, pos,
+                         src, grad);
+  }
   return (int)hipGetLastError();
+}
+
+int vjp_all(const dxtb_basis_t& b, const double* pos, const AdjSrc& src, bool mp, bool scf, double* grad,
+            hipStream_t st) {
+  int e = 0;
+  if ((e = launch_vjp<0, 0>(b, pos, src, mp, scf, grad, st))) return e;
+  if ((e = launch_vjp<0, 1>(b, pos, src, mp, scf, grad, st))) return e;
+  if ((e = launch_vjp<0, 2>(b, pos, src, mp, scf, grad, st))) return e;
+  if ((e = launch_vjp<1, 0>(b, pos, src, mp, scf, grad, st))) return e;
+  if ((e = launch_vjp<1, 1>(b, pos, src, mp, scf, grad, st))) return e;
+  if ((e = launch_vjp<1, 2>(b, pos, src, mp, scf, grad, st))) return e;
+  if ((e = launch_vjp<2, 0>(b, pos, src, mp, scf, grad, st))) return e;
+  if ((e = launch_vjp<2, 1>(b, pos, src, mp, scf, grad, st))) return e;
+  if ((e = launch_vjp<2, 2>(b, pos, src, mp, scf, grad, st))) return e;
+  return 0;
 }
 
 bool basis_ok(const dxtb_basis_t* b) {
@@ -452,16 +517,15 @@ extern "C" int dxtb_multipole_fwd(const dxtb_basis_t* bas, const double* pos, co
 extern "C" int dxtb_multipole_vjp(const dxtb_basis_t* bas, const double* pos, const double* adjS,
                                   const double* adjD, const double* adjQ, double* grad, hipStream_t stream) {
   if (!basis_ok(bas) || pos == nullptr || adjS == nullptr || grad == nullptr) return DXTB_EINVAL;
-  const dxtb_basis_t& b = *bas;
-  int e = 0;
-  if ((e = launch_vjp<0, 0>(b, pos, adjS, adjD, adjQ, grad, stream))) return e;
-  if ((e = launch_vjp<0, 1>(b, pos, adjS, adjD, adjQ, grad, stream))) return e;
-  if ((e = launch_vjp<0, 2>(b, pos, adjS, adjD, adjQ, grad, stream))) return e;
-  if ((e = launch_vjp<1, 0>(b, pos, adjS, adjD, adjQ, grad, stream))) return e;
-  if ((e = launch_vjp<1, 1>(b, pos, adjS, adjD, adjQ, grad, stream))) return e;
-  if ((e = launch_vjp<1, 2>(b, pos, adjS, adjD, adjQ, grad, stream))) return e;
-  if ((e = launch_vjp<2, 0>(b, pos, adjS, adjD, adjQ, grad, stream))) return e;
-  if ((e = launch_vjp<2, 1>(b, pos, adjS, adjD, adjQ, grad, stream))) return e;
-  if ((e = launch_vjp<2, 2>(b, pos, adjS, adjD, adjQ, grad, stream))) return e;
-  return 0;
+  AdjSrc src{adjS, adjD, adjQ, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  return vjp_all(*bas, pos, src, adjD != nullptr || adjQ != nullptr, false, grad, stream);
+}
+
+extern "C" int dxtb_forces_multipole(const dxtb_basis_t* bas, const double* pos, const double* P, const double* W,
+                                     const double* fsh, const double* vsh, const double* vdp, const double* vqp,
+                                     double* grad, hipStream_t stream) {
+  if (!basis_ok(bas) || !pos || !P || !W || !fsh || !vsh || !grad) return DXTB_EINVAL;
+  if ((vdp == nullptr) != (vqp == nullptr)) return DXTB_EINVAL;
+  AdjSrc src{nullptr, nullptr, nullptr, P, W, fsh, vsh, vdp, vqp};
+  return vjp_all(*bas, pos, src, vdp != nullptr, true, grad, stream);
 }

@@ -1,0 +1,332 @@
+"""
+Batched GFN1/GFN2-xTB self-consistent field on the GPU.
+
+Per geometry (once):  CN, H0 pair factors, repulsion, AES2 radii
+(``dxtb_setup``); S, D, Q and H0 (``dxtb_multipole_fwd``); the orthogonaliser
+X = L^-T of the Cholesky factor of S.
+
+Per SCF iteration, for all active molecules of the batch:
+  Fock build (``dxtb_fock``) -> F' = X^T F X -> eigensolve -> C = X C'
+  -> Fermi occupations (``dxtb_fermi``) -> P = C f C^T
+  -> Mulliken charges and atomic multipoles (``dxtb_charges``)
+  -> ES2/ES3/AES2 potentials (``dxtb_potential``) -> Broyden update.
+
+The self-consistency variable is the potential (the reference's
+``scp_mode="potential"``, ``scf/pure/iterations.py:74-110``): the shell
+potential (monopole, incl. the atom part), the atomic dipole and quadrupole
+potentials.  The mixer is the first ("good") Broyden method of xitorch's
+``broyden1`` (``exlibs/xitorch/_impls/optimize/root/rootsolver.py:42-261``,
+alpha=-0.5, damp=0.5, no line search) applied per molecule, so that a
+molecule's iterates do not depend on the other molecules of the batch.
+A molecule is converged when |dx| < x_tol and |y| < f_tol for its own
+vector; converged molecules are frozen while the rest keeps iterating.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib, param
+from .basis import BatchBasis
+from .integrals import multipole_forward
+from .params_dev import device_params
+
+PAD_DIAG = 1.0e3  # Fock diagonal of padding AOs: above every real level
+
+
+@dataclass
+class SCFOptions:
+    etemp: float = 300.0
+    f_tol: float = 1e-10
+    x_tol: float = 1e-10
+    maxiter: int = 250
+    damp: float = 0.5
+    alpha: float = -0.5
+    eigensolver: str = "torch"  # "torch": batched rocSOLVER eigh on the device
+
+
+class GeometryState:
+    """Everything that depends on the geometry but not on the density."""
+
+    def __init__(self, bb: BatchBasis, positions: torch.Tensor, charges=None, opts: SCFOptions | None = None):
+        self.bb = bb
+        self.opts = opts or SCFOptions()
+        self.gfn = bb.gfn
+        self.mp = bb.gfn == "gfn2"
+        dev = positions.device
+        self.dev = dev
+        self.lib = _lib.load()
+        self.dp = device_params(bb.gfn, dev)
+        self.bs = _lib.basis_struct(bb)
+        self.pos = positions.detach().reshape(bb.nbatch, bb.nat_max, 3).contiguous()
+        d = bb.dev
+        B, na, ns, n = bb.nbatch, bb.nat_max, bb.nsh_max, bb.nao_max
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.cn = torch.empty((B, na), **f64)
+        self.fsh = torch.empty((B, ns, ns), **f64)
+        self.dfsh = torch.empty((B, ns, ns), **f64)
+        self.hfsh = torch.empty((B, ns, ns), **f64)
+        self.erep = torch.empty((B, na), **f64)
+        self.mrad = torch.empty((B, na), **f64) if self.mp else None
+        st = _lib.stream_ptr(dev)
+        rc = self.lib.dxtb_setup(self.bs, self.dp.struct, _lib.ptr(d["numbers_i"]), _lib.ptr(self.pos),
+                                 _lib.ptr(d["nsh"]), _lib.ptr(self.cn), _lib.ptr(self.fsh), _lib.ptr(self.dfsh),
+                                 _lib.ptr(self.hfsh), _lib.ptr(self.erep), _lib.ptr(self.mrad), st)
+        _lib.check(rc, "dxtb_setup")
+        self.S, self.D, self.Q, self.H0 = multipole_forward(bb, self.pos, self.mp, fsh=self.fsh)
+        # orthogonaliser X = L^-T with unit diagonal on padding
+        spad = self.S.clone()
+        pad = torch.arange(n, device=dev)[None, :] >= d["nao"][:, None]
+        spad.diagonal(dim1=-2, dim2=-1).copy_(torch.where(pad, 1.0, spad.diagonal(dim1=-2, dim2=-1)))
+        L = torch.linalg.cholesky(spad)
+        eye = torch.eye(n, **f64).expand(B, n, n)
+        self.X = torch.linalg.solve_triangular(L, eye, upper=False).transpose(-1, -2).contiguous()
+        # reference occupations and electron counts (scf/iterator.py:147-195)
+        tab_occ = _refocc_table(bb.gfn)
+        n0 = np.zeros((B, n))
+        nel = np.zeros(B)
+        for b in range(B):
+            u = bb.sh_ush[b, : bb.nsh[b]]
+            occ = tab_occ[u]
+            nel[b] = occ.sum()
+            n0[b, : bb.nao[b]] = np.repeat(occ / (2 * bb.sh_l[b, : bb.nsh[b]] + 1), 2 * bb.sh_l[b, : bb.nsh[b]] + 1)
+        chg = np.zeros(B) if charges is None else np.asarray(
+            charges.cpu() if torch.is_tensor(charges) else charges, dtype=np.float64).reshape(B)
+        nel = nel - chg
+        # alpha/beta split without explicit spin (wavefunction/filling.py:180-259)
+        nint = np.floor(nel + 0.5)
+        nuhf = np.remainder(nint, 2)
+        diff = np.minimum(nuhf, nel)
+        nb = (nel - diff) / 2.0
+        self.nab = torch.as_tensor(np.stack([nb + diff, nb], 1), **f64).contiguous()
+        self.n0 = torch.as_tensor(n0, **f64).contiguous()
+        self.kt = self.opts.etemp * param.KELVIN2AU
+        self.nvec = ns + (9 * na if self.mp else 0)
+        self.valid_sh = (torch.arange(ns, device=dev)[None, :] < d["nsh"][:, None]).double()
+
+    # ---- one application of the SCF map -----------------------------------
+    def unpack(self, x):
+        B, na, ns = self.bb.nbatch, self.bb.nat_max, self.bb.nsh_max
+        vsh = x[:, :ns].contiguous()
+        if not self.mp:
+            return vsh, None, None
+        vdp = x[:, ns : ns + 3 * na].reshape(B, na, 3).contiguous()
+        vqp = x[:, ns + 3 * na :].reshape(B, na, 6).contiguous()
+        return vsh, vdp, vqp
+
+    def fock(self, x):
+        vsh, vdp, vqp = self.unpack(x)
+        bb, d = self.bb, self.bb.dev
+        zero_at = torch.zeros((bb.nbatch, bb.nat_max), dtype=torch.float64, device=self.dev)
+        F = torch.empty_like(self.S)
+        rc = self.lib.dxtb_fock(self.bs, _lib.ptr(d["nao"]), _lib.ptr(d["ao_shell_i"]), _lib.ptr(d["ao_atom_i"]),
+                                _lib.ptr(self.pos), _lib.ptr(self.H0), _lib.ptr(self.S), _lib.ptr(self.D),
+                                _lib.ptr(self.Q), _lib.ptr(vsh), _lib.ptr(zero_at), _lib.ptr(vdp), _lib.ptr(vqp),
+                                PAD_DIAG, _lib.ptr(F), _lib.stream_ptr(self.dev))
+        _lib.check(rc, "dxtb_fock")
+        return F
+
+    def solve(self, F):
+        Fp = self.X.transpose(-1, -2) @ F @ self.X
+        e, Cp = torch.linalg.eigh(Fp)
+        C = self.X @ Cp
+        occ = torch.empty((self.bb.nbatch, 2, self.bb.nao_max), dtype=torch.float64, device=self.dev)
+        ts = torch.empty(self.bb.nbatch, dtype=torch.float64, device=self.dev)
+        e = e.contiguous()
+        rc = self.lib.dxtb_fermi(self.bb.nbatch, self.bb.nao_max, _lib.ptr(self.bb.dev["nao"]), _lib.ptr(e),
+                                 _lib.ptr(self.nab), self.kt, _lib.ptr(occ), _lib.ptr(ts),
+                                 _lib.stream_ptr(self.dev))
+        _lib.check(rc, "dxtb_fermi")
+        f = occ.sum(1)
+        P = (C * f.unsqueeze(-2)) @ C.transpose(-1, -2)
+        return P.contiguous(), e, C, occ, ts
+
+    def charges(self, P):
+        bb, d = self.bb, self.bb.dev
+        B, na, ns, n = bb.nbatch, bb.nat_max, bb.nsh_max, bb.nao_max
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        qsh = torch.empty((B, ns), **f64)
+        qat = torch.empty((B, na), **f64)
+        eh0 = torch.empty((B, na), **f64)
+        qdp = torch.empty((B, na, 3), **f64) if self.mp else None
+        qqp = torch.empty((B, na, 6), **f64) if self.mp else None
+        work = torch.empty((B, n, 11), **f64)
+        rc = self.lib.dxtb_charges(self.bs, _lib.ptr(d["nao"]), _lib.ptr(d["nat"]), _lib.ptr(d["at_sh0"]),
+                                   _lib.ptr(d["at_nsh"]), _lib.ptr(self.n0), _lib.ptr(P), _lib.ptr(self.S),
+                                   _lib.ptr(self.H0), _lib.ptr(self.D), _lib.ptr(self.Q), _lib.ptr(qsh),
+                                   _lib.ptr(qat), _lib.ptr(qdp), _lib.ptr(qqp), _lib.ptr(eh0), _lib.ptr(work),
+                                   _lib.stream_ptr(self.dev))
+        _lib.check(rc, "dxtb_charges")
+        return qsh, qat, qdp, qqp, eh0
+
+    def potential(self, qsh, qat, qdp, qqp):
+        bb, d = self.bb, self.bb.dev
+        B, na, ns = bb.nbatch, bb.nat_max, bb.nsh_max
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        vsh = torch.empty((B, ns), **f64)
+        vat = torch.empty((B, na), **f64)
+        esh = torch.empty((B, ns), **f64)
+        eat = torch.empty((B, na), **f64)
+        vdp = torch.empty((B, na, 3), **f64) if self.mp else None
+        vqp = torch.empty((B, na, 6), **f64) if self.mp else None
+        rc = self.lib.dxtb_potential(self.bs, self.dp.struct, _lib.ptr(d["numbers_i"]), _lib.ptr(self.pos),
+                                     _lib.ptr(d["nsh"]), _lib.ptr(self.mrad), _lib.ptr(qsh), _lib.ptr(qat),
+                                     _lib.ptr(qdp), _lib.ptr(qqp), _lib.ptr(vsh), _lib.ptr(vat), _lib.ptr(vdp),
+                                     _lib.ptr(vqp), _lib.ptr(esh), _lib.ptr(eat), _lib.stream_ptr(self.dev))
+        _lib.check(rc, "dxtb_potential")
+        # total shell potential (atom part spread to its shells)
+        vtot = (vsh + torch.gather(vat, 1, d["sh_atom_l"])) * self.valid_sh
+        return vtot, vdp, vqp, esh, eat
+
+    def pack(self, vtot, vdp, vqp):
+        if not self.mp:
+            return vtot
+        B = self.bb.nbatch
+        return torch.cat([vtot, vdp.reshape(B, -1), vqp.reshape(B, -1)], 1)
+
+    def g(self, x):
+        P, *_ = self.solve(self.fock(x))
+        qsh, qat, qdp, qqp, _ = self.charges(P)
+        vtot, vdp, vqp, _, _ = self.potential(qsh, qat, qdp, qqp)
+        return self.pack(vtot, vdp, vqp)
+
+
+class BatchedBroyden:
+    """Per-molecule xitorch-style broyden1 (good Broyden, rank-one updates of the
+    inverse Jacobian G; G0 = -alpha I)."""
+
+    def __init__(self, alpha: float, damp: float):
+        self.g0 = -alpha
+        self.damp = damp
+        self.cs: list[torch.Tensor] = []
+        self.ds: list[torch.Tensor] = []
+
+    def mv(self, v):
+        r = self.g0 * v
+        for c, d in zip(self.cs, self.ds):
+            r = r + c * (d * v).sum(-1, keepdim=True)
+        return r
+
+    def rmv(self, v):
+        r = self.g0 * v
+        for c, d in zip(self.cs, self.ds):
+            r = r + d * (c * v).sum(-1, keepdim=True)
+        return r
+
+    def update(self, ddx, ddy, active):
+        v = self.rmv(ddx)
+        den = (ddy * v).sum(-1, keepdim=True)
+        ok = active.unsqueeze(-1) & (den != 0)
+        c = torch.where(ok, ddx - self.mv(ddy), 0.0)
+        d = torch.where(ok, v / torch.where(den == 0, 1.0, den), 0.0)
+        self.cs.append(c)
+        self.ds.append(d)
+
+
+@dataclass
+class SCFResult:
+    energy: torch.Tensor  # (B,) total energy
+    eel: torch.Tensor  # electronic energy (H0 + interactions)
+    ts: torch.Tensor  # -TS term
+    erep: torch.Tensor
+    iterations: np.ndarray
+    converged: np.ndarray
+    density: torch.Tensor
+    emo: torch.Tensor
+    coeffs: torch.Tensor
+    occupation: torch.Tensor
+    potential: tuple
+    charges: tuple
+    state: GeometryState
+
+
+def run_scf(state: GeometryState) -> SCFResult:
+    o = state.opts
+    B = state.bb.nbatch
+    dev = state.dev
+    x = torch.zeros((B, state.nvec), dtype=torch.float64, device=dev)
+    y = x - state.g(x)
+    mix = BatchedBroyden(o.alpha, o.damp)
+    active = torch.ones(B, dtype=torch.bool, device=dev)
+    iters = np.zeros(B, dtype=np.int64)
+    conv = np.zeros(B, dtype=bool)
+    xprev, yprev = x, y
+    for it in range(o.maxiter):
+        dx = -mix.mv(y)
+        dxn = dx.norm(dim=-1)
+        xnew = torch.where(active.unsqueeze(-1), x + o.damp * dx, x)
+        ynew = xnew - state.g(xnew)
+        ynew = torch.where(active.unsqueeze(-1), ynew, y)
+        yn = ynew.norm(dim=-1)
+        mix.update(xnew - xprev, ynew - yprev, active)
+        xprev, yprev = xnew, ynew
+        done = active & (dxn < o.x_tol) & (yn < o.f_tol)
+        d_host = done.cpu().numpy()
+        a_host = active.cpu().numpy()
+        iters[a_host] = it + 1
+        conv |= d_host
+        active = active & ~done
+        x, y = xnew, ynew
+        if not bool(active.any()):
+            break
+    # final state at the converged potential
+    F = state.fock(x)
+    P, e, C, occ, ts = state.solve(F)
+    qsh, qat, qdp, qqp, eh0 = state.charges(P)
+    vtot, vdp, vqp, esh, eat = state.potential(qsh, qat, qdp, qqp)
+    eel = eh0.sum(-1) + esh.sum(-1) + eat.sum(-1)
+    erep = state.erep.sum(-1)
+    energy = erep + eel + ts
+    return SCFResult(energy, eel, ts, erep, iters, conv, P, e, C, occ, (vtot, vdp, vqp), (qsh, qat, qdp, qqp), state)
+
+
+def forces(res: SCFResult) -> torch.Tensor:
+    """-dE/dR (nbatch, nat_max, 3) of the converged SCF (hand-written kernels).
+
+    At self-consistency the free energy is stationary in the density, so the
+    gradient is the explicit derivative at fixed P with the orthonormality
+    constraint carried by the energy-weighted density W = C diag(f e) C^T."""
+    st = res.state
+    bb, d = st.bb, st.bb.dev
+    lib = st.lib
+    f = res.occupation.sum(1)
+    W = ((res.coeffs * (f * res.emo).unsqueeze(-2)) @ res.coeffs.transpose(-1, -2)).contiguous()
+    grad = torch.zeros((bb.nbatch, bb.nat_max, 3), dtype=torch.float64, device=st.dev)
+    vtot, vdp, vqp = res.potential
+    vtot = vtot.contiguous()
+    stream = _lib.stream_ptr(st.dev)
+    rc = lib.dxtb_forces_multipole(st.bs, _lib.ptr(st.pos), _lib.ptr(res.density), _lib.ptr(W),
+                                   _lib.ptr(st.fsh), _lib.ptr(vtot), _lib.ptr(vdp), _lib.ptr(vqp),
+                                   _lib.ptr(grad), stream)
+    _lib.check(rc, "dxtb_forces_multipole")
+    qsh, qat, qdp, qqp = res.charges
+    dEdh = torch.empty((bb.nbatch, bb.nsh_max), dtype=torch.float64, device=st.dev)
+    dEdcn = torch.empty((bb.nbatch, bb.nat_max), dtype=torch.float64, device=st.dev)
+    rc = lib.dxtb_forces_classical(st.bs, st.dp.struct, _lib.ptr(d["numbers_i"]), _lib.ptr(st.pos),
+                                   _lib.ptr(d["nsh"]), _lib.ptr(d["at_sh0"]), _lib.ptr(d["at_nsh"]),
+                                   _lib.ptr(res.density), _lib.ptr(st.S), _lib.ptr(st.cn), _lib.ptr(st.mrad),
+                                   _lib.ptr(st.dfsh), _lib.ptr(st.hfsh), _lib.ptr(qsh), _lib.ptr(qat),
+                                   _lib.ptr(qdp), _lib.ptr(qqp), _lib.ptr(dEdh), _lib.ptr(dEdcn), _lib.ptr(grad),
+                                   stream)
+    _lib.check(rc, "dxtb_forces_classical")
+    return -grad
+
+
+_REFOCC: dict = {}
+
+
+def _refocc_table(gfn: str) -> np.ndarray:
+    if gfn not in _REFOCC:
+        from .basis import shell_table
+
+        tab = shell_table(gfn)
+        par = param.get(gfn)
+        occ = np.zeros(len(tab.l))
+        for u in range(len(tab.l)):
+            z = int(tab.z[u])
+            occ[u] = par.element(z)["refocc"][u - int(tab.first[z])]
+        _REFOCC[gfn] = occ
+    return _REFOCC[gfn]

@@ -55,20 +55,19 @@ def draw_elements(rng: np.random.Generator, nat: int, ratios: dict, charge: int 
 def grow_geometry(rng: np.random.Generator, nat: int, bond=(2.0, 2.9), dmin: float = 3.5,
                   tries: int = 50) -> np.ndarray:
     pos = np.zeros((nat, 3))
+    batch = 16  # candidates drawn together; the first acceptable one is taken
     for k in range(1, nat):
-        placed = False
-        while not placed:
-            parent = int(rng.integers(k))
-            for _ in range(tries):
-                v = rng.normal(size=3)
-                v /= np.linalg.norm(v)
-                cand = pos[parent] + rng.uniform(*bond) * v
-                d = np.linalg.norm(pos[:k] - cand, axis=1)
-                d[parent] = np.inf
-                if d.min() >= dmin:
-                    pos[k] = cand
-                    placed = True
-                    break
+        while True:
+            parent = rng.integers(k, size=batch)
+            v = rng.normal(size=(batch, 3))
+            v /= np.linalg.norm(v, axis=1, keepdims=True)
+            cand = pos[parent] + rng.uniform(*bond, size=(batch, 1)) * v
+            d = np.linalg.norm(pos[None, :k, :] - cand[:, None, :], axis=2)
+            d[np.arange(batch), parent] = np.inf
+            ok = np.nonzero(d.min(1) >= dmin)[0]
+            if len(ok):
+                pos[k] = cand[ok[0]]
+                break
     return pos
 
 

@@ -66,6 +66,115 @@ int dxtb_multipole_fwd(const dxtb_basis_t* bas, const double* pos, const double*
 int dxtb_multipole_vjp(const dxtb_basis_t* bas, const double* pos, const double* adjS,
                        const double* adjD, const double* adjQ, double* grad, hipStream_t stream);
 
+/* Method parameters as device tables.  Per-unique-shell arrays use the row
+ * order of dxtb_basis_t.ush_*; per-element arrays are indexed by the atomic
+ * number (0..86, row 0 unused). */
+typedef struct dxtb_params {
+  int gfn;                 /* 1 or 2 */
+  int es3_shell;           /* 1: shell-resolved third order (GFN2), 0: atom (GFN1) */
+  int es2_average;         /* 0 harmonic, 1 arithmetic, 2 geometric */
+  int rep_light;           /* 1: light-atom repulsion exponent rep_klight for H/He pairs */
+  double enscale, es2_gexp, rep_kexp, rep_klight;
+  double aes_dmp3, aes_dmp5, aes_kexp, aes_shift, aes_rmax;
+  const double* ush_lev;    /* (nush) shell level * EV2AU */
+  const double* ush_kcn;    /* (nush) kcn * EV2AU */
+  const double* ush_shpoly; /* (nush) */
+  const double* ush_hub;    /* (nush) gam(Z) * lgam(shell) */
+  const double* ush_hd3;    /* (nush) GFN2: gam3(Z) * shell scale */
+  const int* ush_valence;   /* (nush) 1 valence / 0 not */
+  const double* hscale;     /* (nush, nush) */
+  const double* z_rad;      /* atomic radii of the H0 polynomial (Bohr) */
+  const double* z_en;       /* electronegativity */
+  const double* z_rcov;     /* D3 covalent radii (Bohr) */
+  const double* kpair;      /* (87, 87) */
+  const double* z_gam3;     /* GFN1 atom third-order */
+  const double* z_arep;
+  const double* z_zeff;
+  const double* z_mprad;
+  const double* z_mpvcn;
+  const double* z_dkernel;
+  const double* z_qkernel;
+} dxtb_params_t;
+
+/* Per-geometry set-up: coordination numbers cn (nbatch, nat_max); H0
+ * shell-pair factor fsh with its radial derivative dfsh and d f / d h
+ * (hfsh), all (nbatch, nsh_max, nsh_max); per-atom repulsion energy erep
+ * (nbatch, nat_max); AES2 multipole radii mrad (nbatch, nat_max, GFN2 only,
+ * may be NULL otherwise).
+ * Replaces tad-mctc cn_d3 as called by xtb/gfn1.py:58-64 and xtb/gfn2.py:58-64,
+ * the H0 factors of xtb/base.py:255-372, repulsion/base.py:293-358 and the
+ * mrad of coulomb/multipole.py:320-333.  nsh: (nbatch) shells per molecule. */
+int dxtb_setup(const dxtb_basis_t* bas, const dxtb_params_t* par, const int* numbers, const double* pos,
+               const int* nsh, double* cn, double* fsh, double* dfsh, double* hfsh, double* erep,
+               double* mrad, hipStream_t stream);
+
+/* Fock matrix F (nbatch, nao_max, nao_max) from H0, S, ket-centred D and Q
+ * and the shell/atom potentials vsh (nbatch, nsh_max), vat (nbatch, nat_max),
+ * vdp (nbatch, nat_max, 3), vqp (nbatch, nat_max, 6).  D/Q/vdp/vqp may be
+ * NULL (GFN1).  Padding rows/columns get F_ii = pad_diag.
+ * Replaces scf/pure/conversions.py:255-297 (potential_to_hamiltonian).
+ * ao_shell/ao_atom: (nbatch, nao_max) maps, nao: (nbatch). */
+int dxtb_fock(const dxtb_basis_t* bas, const int* nao, const int* ao_shell, const int* ao_atom,
+              const double* pos, const double* H0, const double* S, const double* D, const double* Q,
+              const double* vsh, const double* vat, const double* vdp, const double* vqp, double pad_diag,
+              double* F, hipStream_t stream);
+
+/* Mulliken populations and atomic multipoles of a density P:
+ * qsh (nbatch, nsh_max) = n0-sum shell charges, qat (nbatch, nat_max),
+ * qdp (nbatch, nat_max, 3), qqp (nbatch, nat_max, 6) and the per-atom
+ * band energy eh0 (nbatch, nat_max) = sum diag(P H0).  n0: (nbatch, nao_max)
+ * reference occupations.  at_sh0/at_nsh: first shell / shell count per atom.
+ * work: (nbatch, nao_max, 11) scratch.
+ * Replaces scf/pure/conversions.py:203-252 (density_to_charges). */
+int dxtb_charges(const dxtb_basis_t* bas, const int* nao, const int* nat, const int* at_sh0, const int* at_nsh,
+                 const double* n0, const double* P, const double* S, const double* H0, const double* D,
+                 const double* Q, double* qsh, double* qat, double* qdp, double* qqp, double* eh0,
+                 double* work, hipStream_t stream);
+
+/* Potentials and interaction energies of ES2 + ES3 (+ AES2 for GFN2) from
+ * the charges: vsh (nbatch, nsh_max), vat (nbatch, nat_max), vdp, vqp;
+ * per-shell esh and per-atom eat energies.
+ * Replaces components/interactions/base.py:135-181 with coulomb/secondorder.py,
+ * thirdorder.py and multipole.py. */
+int dxtb_potential(const dxtb_basis_t* bas, const dxtb_params_t* par, const int* numbers, const double* pos,
+                   const int* nsh, const double* mrad, const double* qsh, const double* qat, const double* qdp,
+                   const double* qqp, double* vsh, double* vat, double* vdp, double* vqp, double* esh,
+                   double* eat, hipStream_t stream);
+
+/* Fermi-Dirac occupations occ (nbatch, 2, nao_max) of ascending orbital
+ * energies emo (nbatch, nao_max) for nab (nbatch, 2) electrons at kT, and the
+ * electronic free-energy term ts (nbatch) = kT sum f ln f + (1-f) ln(1-f).
+ * Replaces wavefunction/filling.py:428-742 and scf/base.py:554-619. */
+int dxtb_fermi(int nbatch, int nao_max, const int* nao, const double* emo, const double* nab, double kt,
+               double* occ, double* ts, hipStream_t stream);
+
+/* Integral part of the nuclear gradient of the converged SCF free energy:
+ * sum over all AO pairs of adjS dS/dR + adjD dD/dR + adjQ dQ/dR with the
+ * variational adjoints built on the fly from the density P and the
+ * energy-weighted density W (nbatch, nao_max, nao_max):
+ *   adjS = P (fsh - v_ket) - W, adjD = -P vdp(ket atom), adjQ = -P vqp(ket atom).
+ * vsh: total shell potential (nbatch, nsh_max); vdp/vqp NULL for GFN1.
+ * grad (nbatch, nat_max, 3) is accumulated into.  Replaces the autograd
+ * backward of the reference through its integrals (calculators/types/autograd.py:82-193);
+ * the formula is the one of the analytic GFN1 gradient (xtb/gfn1.py:311-313). */
+int dxtb_forces_multipole(const dxtb_basis_t* bas, const double* pos, const double* P, const double* W,
+                          const double* fsh, const double* vsh, const double* vdp, const double* vqp,
+                          double* grad, hipStream_t stream);
+
+/* Remaining gradient terms at fixed charges: H0 distance polynomial and
+ * self-energy (CN) terms, ES2 gamma(R), AES2 matrices and radii (GFN2),
+ * repulsion and the coordination-number chain.  dEdh (nbatch, nsh_max) and
+ * dEdcn (nbatch, nat_max) are workspaces (outputs for inspection); grad is
+ * accumulated into.  Replaces the autograd backward through
+ * xtb/base.py:255-372, coulomb/secondorder.py:799-870, coulomb/multipole.py:270-449,
+ * repulsion/base.py:293-430 and tad-mctc cn_d3. */
+int dxtb_forces_classical(const dxtb_basis_t* bas, const dxtb_params_t* par, const int* numbers,
+                          const double* pos, const int* nsh, const int* at_sh0, const int* at_nsh,
+                          const double* P, const double* S, const double* cn, const double* mrad,
+                          const double* dfsh, const double* hfsh, const double* qsh, const double* qat,
+                          const double* qdp, const double* qqp, double* dEdh, double* dEdcn, double* grad,
+                          hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

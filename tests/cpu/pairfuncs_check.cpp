@@ -1,0 +1,87 @@
+// Finite-difference check of the analytic pair derivatives in
+// dxtb_amd/csrc/pairfuncs.h (built and run on the host by tests/test_pairfuncs.py).
+#include <cmath>
+#include <cstdio>
+#include <random>
+
+#include "../../dxtb_amd/csrc/pairfuncs.h"
+
+using namespace dxtb;
+
+static double maxerr = 0.0;
+static void check(const char* what, double ana, double num, double scale) {
+  const double e = fabs(ana - num) / (1.0 + fabs(scale));
+  if (e > maxerr) maxerr = e;
+  if (e > 1e-7) printf("MISMATCH %s ana=%.12e num=%.12e\n", what, ana, num);
+}
+
+int main() {
+  std::mt19937_64 rng(12345);
+  std::uniform_real_distribution<double> U(-1.0, 1.0);
+  const double h = 1e-6;
+  for (int trial = 0; trial < 200; ++trial) {
+    double r[3] = {2.5 * U(rng), 2.5 * U(rng), 2.5 * U(rng)};
+    r[0] += (r[0] >= 0 ? 1.5 : -1.5);
+    const double mi = 2.0 + U(rng), mj = 2.5 + U(rng);
+    const double qi = 0.4 * U(rng), qj = 0.4 * U(rng);
+    double mui[3], muj[3], thi[6], thj[6];
+    for (int k = 0; k < 3; ++k) mui[k] = 0.3 * U(rng), muj[k] = 0.3 * U(rng);
+    for (int k = 0; k < 6; ++k) thi[k] = 0.5 * U(rng), thj[k] = 0.5 * U(rng);
+    double E, g[3], dm;
+    aes2_pair(r, mi, mj, qi, qj, mui, muj, thi, thj, 3.0, 4.0, E, g, dm);
+    for (int k = 0; k < 3; ++k) {
+      double rp[3] = {r[0], r[1], r[2]}, rm[3] = {r[0], r[1], r[2]};
+      rp[k] += h;
+      rm[k] -= h;
+      double Ep, Em, gg[3], d;
+      aes2_pair(rp, mi, mj, qi, qj, mui, muj, thi, thj, 3.0, 4.0, Ep, gg, d);
+      aes2_pair(rm, mi, mj, qi, qj, mui, muj, thi, thj, 3.0, 4.0, Em, gg, d);
+      check("aes2 dr", g[k], (Ep - Em) / (2 * h), g[k]);
+    }
+    {
+      double Ep, Em, gg[3], d;
+      aes2_pair(r, mi + h, mj, qi, qj, mui, muj, thi, thj, 3.0, 4.0, Ep, gg, d);
+      aes2_pair(r, mi - h, mj, qi, qj, mui, muj, thi, thj, 3.0, 4.0, Em, gg, d);
+      check("aes2 dm", dm, (Ep - Em) / (2 * h), dm);
+    }
+    const double R = 1.5 + 10.0 * (U(rng) + 1.0), r0 = 2.0 + U(rng);
+    for (int gfn = 1; gfn <= 2; ++gfn) {
+      double c, dc, cp, cm, t;
+      cn_count(gfn, R, r0, c, dc);
+      cn_count(gfn, R + h, r0, cp, t);
+      cn_count(gfn, R - h, r0, cm, t);
+      check("cn", dc, (cp - cm) / (2 * h), dc);
+    }
+    {
+      double p, dp, pp, pm, t;
+      h0_poly(R, 3.1, 0.3 * U(rng), 0.2 * U(rng), p, dp);
+      const double ka = 0.13, kb = -0.07;
+      h0_poly(R, 3.1, ka, kb, p, dp);
+      h0_poly(R + h, 3.1, ka, kb, pp, t);
+      h0_poly(R - h, 3.1, ka, kb, pm, t);
+      check("h0poly", dp, (pp - pm) / (2 * h), dp);
+    }
+    {
+      double g0, dg, gp, gm, t;
+      es2_gamma(false, R, 0.45, g0, dg);
+      es2_gamma(false, R + h, 0.45, gp, t);
+      es2_gamma(false, R - h, 0.45, gm, t);
+      check("gamma", dg, (gp - gm) / (2 * h), dg);
+    }
+    {
+      double e, de, ep, em, t;
+      rep_pair(R, 1.7, 1.5, 20.0, e, de);
+      rep_pair(R + h, 1.7, 1.5, 20.0, ep, t);
+      rep_pair(R - h, 1.7, 1.5, 20.0, em, t);
+      check("rep", de, (ep - em) / (2 * h), de);
+    }
+    {
+      const double cn = 3.0 * (U(rng) + 1.0);
+      const double a = dmrad_dcn(cn, 3.0, 2.0, 4.0, 1.2, 5.0);
+      auto mr = [](double c) { return 3.0 + (5.0 - 3.0) / (1.0 + exp(-4.0 * (c - 2.0 - 1.2))); };
+      check("mrad", a, (mr(cn + h) - mr(cn - h)) / (2 * h), a);
+    }
+  }
+  printf("maxerr %.3e\n", maxerr);
+  return maxerr < 1e-7 ? 0 : 1;
+}

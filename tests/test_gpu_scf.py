@@ -1,0 +1,89 @@
+"""
+GPU SCF energies against the oracle (|dE| < 1e-8 Eh),
+batched and single, GFN1 and GFN2, and against the reference's tblite GFN1
+SCF energies at the reference's own tolerance.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from dxtb_amd import basis as hbasis
+from dxtb_amd import scf as hscf
+from dxtb_amd import workloads
+from oracle import scf as oscf
+
+pytestmark = pytest.mark.gpu
+
+ETOL = 1e-8
+
+
+def _run(mols, gfn, dev, charges=None):
+    N, P = workloads.pad([m[0] for m in mols], [m[1] for m in mols])
+    bb = hbasis.build_batch(N, gfn, dev)
+    st = hscf.GeometryState(bb, torch.as_tensor(P, dtype=torch.float64, device=dev), charges)
+    return hscf.run_scf(st)
+
+
+@pytest.mark.parametrize("gfn", ["gfn1", "gfn2"])
+@pytest.mark.parametrize("name", ["H2", "LiH", "H2O", "CH4", "SiH4", "MB16_43_01"])
+def test_single_vs_oracle(cuda, geometries, gfn, name):
+    m = geometries[name]
+    print(f"case {gfn} {name}", flush=True)
+    res = _run([(np.array(m["numbers"]), np.array(m["positions"]))], gfn, cuda)
+    assert res.converged.all()
+    ref = oscf.singlepoint(m["numbers"], torch.tensor(m["positions"], dtype=torch.float64), gfn, tol=1e-11)
+    assert abs(res.energy[0].item() - ref["energy"].item()) < ETOL
+    assert abs(res.eel[0].item() - ref["eel"].item()) < ETOL
+
+
+@pytest.mark.parametrize("name", ["H2", "LiH", "H2O", "CH4", "SiH4"])
+def test_gfn1_vs_tblite(cuda, geometries, refvals, name):
+    m = geometries[name]
+    res = _run([(np.array(m["numbers"]), np.array(m["positions"]))], "gfn1", cuda)
+    ref = refvals["scf"][name]["egfn1"]
+    tol = math.sqrt(np.finfo(np.float64).eps) * 10
+    assert abs(res.eel[0].item() - ref) <= max(tol, tol * abs(ref))
+
+
+@pytest.mark.parametrize("gfn", ["gfn1", "gfn2"])
+def test_batch_equals_single(cuda, geometries, gfn):
+    """Padded batch of molecules of different size: each result matches the
+    oracle and the same molecule run alone (per-molecule mixing)."""
+    names = ["H2O", "MB16_43_01", "LiH", "SiH4", "CH4"]
+    mols = [(np.array(geometries[k]["numbers"]), np.array(geometries[k]["positions"])) for k in names]
+    print(f"case batch {gfn}", flush=True)
+    res = _run(mols, gfn, cuda)
+    assert res.converged.all()
+    for i, (z, p) in enumerate(mols):
+        ref = oscf.singlepoint(z.tolist(), torch.tensor(p, dtype=torch.float64), gfn, tol=1e-11)
+        assert abs(res.energy[i].item() - ref["energy"].item()) < ETOL
+        alone = _run([(z, p)], gfn, cuda)
+        assert abs(res.energy[i].item() - alone.energy[0].item()) < 1e-9
+
+
+def test_charged_molecule(cuda, geometries):
+    m = geometries["H2O"]
+    # H2O+ has an odd electron count: alpha/beta split with one unpaired electron
+    res = _run([(np.array(m["numbers"]), np.array(m["positions"]))], "gfn2", cuda, charges=[1.0])
+    ref = oscf.singlepoint(m["numbers"], torch.tensor(m["positions"], dtype=torch.float64), "gfn2",
+                           charge=1.0, tol=1e-11)
+    assert abs(res.energy[0].item() - ref["energy"].item()) < ETOL
+
+
+@pytest.mark.parametrize("gfn", ["gfn1", "gfn2"])
+@pytest.mark.parametrize("name", ["H2O", "SiH4", "MB16_43_01"])
+def test_forces_vs_oracle(cuda, geometries, gfn, name):
+    """Hand-written force kernels against the oracle's variational forces
+    (themselves pinned by finite differences) at 1e-6 Eh/Bohr; expected ~1e-9."""
+    m = geometries[name]
+    print(f"case forces {gfn} {name}", flush=True)
+    res = _run([(np.array(m["numbers"]), np.array(m["positions"]))], gfn, cuda)
+    f = hscf.forces(res)[0].cpu()
+    ref = oscf.singlepoint(m["numbers"], torch.tensor(m["positions"], dtype=torch.float64), gfn, tol=1e-11,
+                           forces=True)["forces"]
+    torch.testing.assert_close(f, ref, atol=1e-7, rtol=0)
