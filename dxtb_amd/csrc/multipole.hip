@@ -130,7 +130,8 @@ __global__ __launch_bounds__(256) void k_multipole_fwd(dxtb_basis_t bs, int off,
                                                        const double* __restrict__ pos,
                                                        const double* __restrict__ fsh,
                                                        double* __restrict__ S, double* __restrict__ D,
-                                                       double* __restrict__ Q, double* __restrict__ H0) {
+                                                       double* __restrict__ Q, double* __restrict__ H0,
+                                                       int traceless) {
   constexpr int NA = 2 * LA + 1, NB = 2 * LB + 1, NCB = Cart<LB>::n;
   constexpr int NI = LA + 1, NJ = LB + (MP ? 3 : 1);
   constexpr int NCOMB = (LA == 2) ? 3 : 1;
@@ -229,13 +230,22 @@ __global__ __launch_bounds__(256) void k_multipole_fwd(dxtb_basis_t bs, int off,
       }
       if (Q != nullptr) {
         const size_t qb = (size_t)c.b * 6 * nn + (size_t)c.row * n + c.col0 + mb;
-        const double tr = 0.5 * (v[4] + v[7] + v[9]);
-        Q[qb] = 1.5 * v[4] - tr;
-        Q[qb + nn] = 1.5 * v[5];
-        Q[qb + 2 * nn] = 1.5 * v[7] - tr;
-        Q[qb + 3 * nn] = 1.5 * v[6];
-        Q[qb + 4 * nn] = 1.5 * v[8];
-        Q[qb + 5 * nn] = 1.5 * v[9] - tr;
+        if (traceless) {
+          const double tr = 0.5 * (v[4] + v[7] + v[9]);
+          Q[qb] = 1.5 * v[4] - tr;
+          Q[qb + nn] = 1.5 * v[5];
+          Q[qb + 2 * nn] = 1.5 * v[7] - tr;
+          Q[qb + 3 * nn] = 1.5 * v[6];
+          Q[qb + 4 * nn] = 1.5 * v[8];
+          Q[qb + 5 * nn] = 1.5 * v[9] - tr;
+        } else {
+          Q[qb] = v[4];
+          Q[qb + nn] = v[5];
+          Q[qb + 2 * nn] = v[7];
+          Q[qb + 3 * nn] = v[6];
+          Q[qb + 4 * nn] = v[8];
+          Q[qb + 5 * nn] = v[9];
+        }
       }
     }
   }
@@ -425,7 +435,7 @@ __global__ __launch_bounds__(256) void k_multipole_vjp(dxtb_basis_t bs, int off,
 
 template <int LA, int LB>
 int launch_fwd(const dxtb_basis_t& bs, const double* pos, const double* fsh, double* S, double* D,
-               double* Q, double* H0, hipStream_t st) {
+               double* Q, double* H0, int tl, hipStream_t st) {
   const int c = 3 * LA + LB;
   const int off = bs.class_off[c], cnt = bs.class_off[c + 1] - off;
   if (cnt <= 0) return 0;
@@ -434,10 +444,10 @@ int launch_fwd(const dxtb_basis_t& bs, const double* pos, const double* fsh, dou
   const dim3 grid((unsigned)((threads + blk - 1) / blk));
   if (D != nullptr || Q != nullptr)
     hipLaunchKernelGGL((k_multipole_fwd<LA, LB, true>), grid, dim3(blk), 0, st, bs, off, cnt, pos, fsh, S,
-                       D, Q, H0);
+                       D, Q, H0, tl);
   else
     hipLaunchKernelGGL((k_multipole_fwd<LA, LB, false>), grid, dim3(blk), 0, st, bs, off, cnt, pos, fsh,
-                       S, D, Q, H0);
+                       S, D, Q, H0, tl);
   return (int)hipGetLastError();
 }
 
@@ -497,20 +507,20 @@ bool basis_ok(const dxtb_basis_t* b) {
 }  // namespace
 
 extern "C" int dxtb_multipole_fwd(const dxtb_basis_t* bas, const double* pos, const double* fsh, double* S,
-                                  double* D, double* Q, double* H0, hipStream_t stream) {
+                                  double* D, double* Q, double* H0, int traceless, hipStream_t stream) {
   if (!basis_ok(bas) || pos == nullptr || S == nullptr) return DXTB_EINVAL;
   if (H0 != nullptr && fsh == nullptr) return DXTB_EINVAL;
   const dxtb_basis_t& b = *bas;
   int e = 0;
-  if ((e = launch_fwd<0, 0>(b, pos, fsh, S, D, Q, H0, stream))) return e;
-  if ((e = launch_fwd<0, 1>(b, pos, fsh, S, D, Q, H0, stream))) return e;
-  if ((e = launch_fwd<0, 2>(b, pos, fsh, S, D, Q, H0, stream))) return e;
-  if ((e = launch_fwd<1, 0>(b, pos, fsh, S, D, Q, H0, stream))) return e;
-  if ((e = launch_fwd<1, 1>(b, pos, fsh, S, D, Q, H0, stream))) return e;
-  if ((e = launch_fwd<1, 2>(b, pos, fsh, S, D, Q, H0, stream))) return e;
-  if ((e = launch_fwd<2, 0>(b, pos, fsh, S, D, Q, H0, stream))) return e;
-  if ((e = launch_fwd<2, 1>(b, pos, fsh, S, D, Q, H0, stream))) return e;
-  if ((e = launch_fwd<2, 2>(b, pos, fsh, S, D, Q, H0, stream))) return e;
+  if ((e = launch_fwd<0, 0>(b, pos, fsh, S, D, Q, H0, traceless, stream))) return e;
+  if ((e = launch_fwd<0, 1>(b, pos, fsh, S, D, Q, H0, traceless, stream))) return e;
+  if ((e = launch_fwd<0, 2>(b, pos, fsh, S, D, Q, H0, traceless, stream))) return e;
+  if ((e = launch_fwd<1, 0>(b, pos, fsh, S, D, Q, H0, traceless, stream))) return e;
+  if ((e = launch_fwd<1, 1>(b, pos, fsh, S, D, Q, H0, traceless, stream))) return e;
+  if ((e = launch_fwd<1, 2>(b, pos, fsh, S, D, Q, H0, traceless, stream))) return e;
+  if ((e = launch_fwd<2, 0>(b, pos, fsh, S, D, Q, H0, traceless, stream))) return e;
+  if ((e = launch_fwd<2, 1>(b, pos, fsh, S, D, Q, H0, traceless, stream))) return e;
+  if ((e = launch_fwd<2, 2>(b, pos, fsh, S, D, Q, H0, traceless, stream))) return e;
   return 0;
 }
 

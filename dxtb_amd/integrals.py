@@ -28,8 +28,9 @@ def _check_positions(bb: BatchBasis, positions: torch.Tensor) -> torch.Tensor:
 
 
 def multipole_forward(bb: BatchBasis, positions: torch.Tensor, multipoles: bool = True,
-                      fsh: torch.Tensor | None = None):
-    """S, D, Q (None if ``multipoles`` is False) and H0 (if ``fsh`` is given)."""
+                      fsh: torch.Tensor | None = None, traceless: bool = True):
+    """S, D, Q (None if ``multipoles`` is False) and H0 (if ``fsh`` is given).
+    ``traceless=False`` returns the raw ket-centred second moments."""
     lib = _lib.load()
     pos = _check_positions(bb, positions)
     dev = pos.device
@@ -42,7 +43,7 @@ def multipole_forward(bb: BatchBasis, positions: torch.Tensor, multipoles: bool 
     st = _lib.basis_struct(bb)
     rc = lib.dxtb_multipole_fwd(
         st, _lib.ptr(pos), _lib.ptr(fsh), _lib.ptr(S), _lib.ptr(D), _lib.ptr(Q), _lib.ptr(H0),
-        _lib.stream_ptr(dev),
+        1 if traceless else 0, _lib.stream_ptr(dev),
     )
     _lib.check(rc, "dxtb_multipole_fwd")
     return S, D, Q, H0

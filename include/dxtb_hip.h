@@ -55,9 +55,11 @@ typedef struct dxtb_basis {
  * pipeline.py:151-235, kernels/os.py:61-163) followed by the container
  * post-processing (integral/container.py:182-535: normalisation, r0->rj
  * shift, traceless) and the H0 build (xtb/base.py:255-372).  D, Q, H0 and fsh
- * may be NULL.  Outputs must be zeroed by the caller (padding is not written). */
+ * may be NULL.  traceless = 0 writes the raw ket-centred second moments in the
+ * same component order instead of the traceless form.  Outputs must be zeroed
+ * by the caller (padding is not written). */
 int dxtb_multipole_fwd(const dxtb_basis_t* bas, const double* pos, const double* fsh,
-                       double* S, double* D, double* Q, double* H0, hipStream_t stream);
+                       double* S, double* D, double* Q, double* H0, int traceless, hipStream_t stream);
 
 /* Nuclear gradient of sum(adjS*S) + sum(adjD*D) + sum(adjQ*Q) (the VJP of
  * dxtb_multipole_fwd).  adjD/adjQ may be NULL.  grad (nbatch, nat_max, 3) is

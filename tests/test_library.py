@@ -41,5 +41,5 @@ def test_invalid_arguments_rejected_without_gpu():
     """Argument checks run on the host before any launch."""
     lib = _lib.load()
     st = _lib.DxtbBasis()  # all zero: invalid
-    assert lib.dxtb_multipole_fwd(st, None, None, None, None, None, None, None) == _lib.DXTB_EINVAL
+    assert lib.dxtb_multipole_fwd(st, None, None, None, None, None, None, 1, None) == _lib.DXTB_EINVAL
     assert lib.dxtb_multipole_vjp(st, None, None, None, None, None, None) == _lib.DXTB_EINVAL
