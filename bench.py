@@ -161,7 +161,11 @@ def bench_gfn2_batch(args, dev):
 
     nums, poss, _ = workloads.config_batch(2, count=args.nmol)
     tab = hbasis.shell_table("gfn2")
-    nao = np.array([int(sum(2 * tab.l[tab.first[z] + k] + 1 for k in range(tab.count[z])) for z in zz) for zz in nums])
+    ao_per_z = np.zeros(len(tab.first), dtype=np.int64)
+    for z in range(1, len(tab.first)):
+        if tab.first[z] >= 0:
+            ao_per_z[z] = int((2 * tab.l[tab.first[z] : tab.first[z] + tab.count[z]] + 1).sum())
+    nao = np.array([int(ao_per_z[np.asarray(zz)].sum()) for zz in nums])
     order = np.argsort(nao, kind="stable")
     buckets = [order[i : i + args.bucket] for i in range(0, len(order), args.bucket)]
     prepared = []
@@ -181,6 +185,9 @@ def bench_gfn2_batch(args, dev):
             dt = time.perf_counter() - t0
             if record is not None:
                 record.append((len(idx), dt, int(res.iterations.max()), bool(res.converged.all())))
+                print(f"bucket n={len(idx)} nao_max={bb.nao_max} {dt*1e3:.1f} ms "
+                      f"iters max={int(res.iterations.max())} mean={res.iterations.mean():.1f}",
+                      file=sys.stderr, flush=True)
             out.append((res.energy, f))
         return out
 

@@ -81,6 +81,8 @@ _SIGNATURES: dict[str, list] = {
     "dxtb_potential": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 14 + [c_void_p],
     "dxtb_fermi": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p],
     "dxtb_forces_multipole": [POINTER(DxtbBasis)] + [c_void_p] * 9,
+    "dxtb_jacobi_max_n": [],
+    "dxtb_jacobi_eigh": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p],
     "dxtb_forces_classical": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 19,
 }
 

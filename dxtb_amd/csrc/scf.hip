@@ -176,6 +176,19 @@ __device__ double block_sum(double v, double* sh) {
   return r;
 }
 
+__device__ double block_sum_max(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = sh[0];
+#pragma unroll
+  for (int k = 1; k < FB / 64; ++k) r = fmax(r, sh[k]);
+  return r;
+}
+
 __device__ __forceinline__ double fermi_fn(double e, double mu, double kt) {
   const double x = (e - mu) / kt;
   return x > 0.0 ? exp(-x) / (1.0 + exp(-x)) : 1.0 / (1.0 + exp(x));
@@ -200,7 +213,15 @@ __global__ __launch_bounds__(FB) void k_fermi(int n, const int* __restrict__ nao
       for (int i = threadIdx.x; i < n; i += FB) o[i] = 0.0;
       continue;
     }
-    double lo = e[0] - 1.0, hi = e[nb - 1] + 1.0;
+    // bracket from the extreme levels (the eigensolver need not sort them)
+    double emin = 1e300, emax = -1e300;
+    for (int i = threadIdx.x; i < nb; i += FB) {
+      emin = fmin(emin, e[i]);
+      emax = fmax(emax, e[i]);
+    }
+    emin = -block_sum_max(-emin, sh);
+    emax = block_sum_max(emax, sh);
+    double lo = emin - 1.0, hi = emax + 1.0;
     double mu = 0.5 * (lo + hi);
     for (int it = 0; it < 200; ++it) {
       double s = 0.0, ds = 0.0;

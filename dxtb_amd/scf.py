@@ -40,12 +40,12 @@ PAD_DIAG = 1.0e3  # Fock diagonal of padding AOs: above every real level
 @dataclass
 class SCFOptions:
     etemp: float = 300.0
-    f_tol: float = 1e-10
-    x_tol: float = 1e-10
+    f_tol: float = 1e-8
+    x_tol: float = 1e-8
     maxiter: int = 250
     damp: float = 0.5
     alpha: float = -0.5
-    eigensolver: str = "torch"  # "torch": batched rocSOLVER eigh on the device
+    eigensolver: str = "jacobi"  # "jacobi": hand-written LDS Jacobi for nao <= 128, else device eigh
 
 
 class GeometryState:
@@ -106,6 +106,7 @@ class GeometryState:
         self.kt = self.opts.etemp * param.KELVIN2AU
         self.nvec = ns + (9 * na if self.mp else 0)
         self.valid_sh = (torch.arange(ns, device=dev)[None, :] < d["nsh"][:, None]).double()
+        self.C = None  # previous orbitals (warm start of the Jacobi solver)
 
     # ---- one application of the SCF map -----------------------------------
     def unpack(self, x):
@@ -130,9 +131,28 @@ class GeometryState:
         return F
 
     def solve(self, F):
-        Fp = self.X.transpose(-1, -2) @ F @ self.X
-        e, Cp = torch.linalg.eigh(Fp)
-        C = self.X @ Cp
+        """Orbitals, occupations and density of a Fock matrix.
+
+        Small bases: the hand-written LDS Jacobi solver on A = C^T F C with C
+        the previous iteration's orbitals (S-orthonormal, so A is the Fock
+        matrix in the old orbital basis and nearly diagonal; the first
+        iteration uses C = X = L^-T); the new orbitals are C V.  Larger bases:
+        X^T F X and the device eigh (rocSOLVER)."""
+        n = self.bb.nao_max
+        if self.opts.eigensolver == "jacobi" and n <= self.lib.dxtb_jacobi_max_n():
+            Cb = self.X if self.C is None else self.C
+            A = (Cb.transpose(-1, -2) @ F @ Cb).contiguous()
+            V = torch.empty_like(A)
+            e = torch.empty((self.bb.nbatch, n), dtype=torch.float64, device=self.dev)
+            rc = self.lib.dxtb_jacobi_eigh(self.bb.nbatch, n, _lib.ptr(self.bb.dev["nao"]), _lib.ptr(A),
+                                           _lib.ptr(V), _lib.ptr(e), 50, 1e-14, None, _lib.stream_ptr(self.dev))
+            _lib.check(rc, "dxtb_jacobi_eigh")
+            C = Cb @ V
+            self.C = C
+        else:
+            Fp = self.X.transpose(-1, -2) @ F @ self.X
+            e, Cp = torch.linalg.eigh(Fp)
+            C = self.X @ Cp
         occ = torch.empty((self.bb.nbatch, 2, self.bb.nao_max), dtype=torch.float64, device=self.dev)
         ts = torch.empty(self.bb.nbatch, dtype=torch.float64, device=self.dev)
         e = e.contiguous()
@@ -196,34 +216,40 @@ class GeometryState:
 
 class BatchedBroyden:
     """Per-molecule xitorch-style broyden1 (good Broyden, rank-one updates of the
-    inverse Jacobian G; G0 = -alpha I)."""
+    inverse Jacobian G = -alpha I + sum_k c_k d_k^T).  The update vectors live
+    in preallocated (B, H, L) buffers so that a product with G is two batched
+    GEMVs whatever the history length."""
 
-    def __init__(self, alpha: float, damp: float):
+    def __init__(self, alpha: float, damp: float, B: int, L: int, H: int, device):
         self.g0 = -alpha
         self.damp = damp
-        self.cs: list[torch.Tensor] = []
-        self.ds: list[torch.Tensor] = []
+        self.C = torch.zeros((B, H, L), dtype=torch.float64, device=device)
+        self.D = torch.zeros((B, H, L), dtype=torch.float64, device=device)
+        self.k = 0
+
+    def _apply(self, A, Bm, v):
+        r = self.g0 * v
+        if self.k == 0:
+            return r
+        coef = torch.bmm(Bm[:, : self.k], v.unsqueeze(-1))  # (B, k, 1)
+        return r + torch.bmm(A[:, : self.k].transpose(1, 2), coef).squeeze(-1)
 
     def mv(self, v):
-        r = self.g0 * v
-        for c, d in zip(self.cs, self.ds):
-            r = r + c * (d * v).sum(-1, keepdim=True)
-        return r
+        return self._apply(self.C, self.D, v)
 
     def rmv(self, v):
-        r = self.g0 * v
-        for c, d in zip(self.cs, self.ds):
-            r = r + d * (c * v).sum(-1, keepdim=True)
-        return r
+        return self._apply(self.D, self.C, v)
 
     def update(self, ddx, ddy, active):
+        if self.k >= self.C.shape[1]:
+            # xitorch "restart" reduction: forget the history
+            self.k = 0
         v = self.rmv(ddx)
         den = (ddy * v).sum(-1, keepdim=True)
         ok = active.unsqueeze(-1) & (den != 0)
-        c = torch.where(ok, ddx - self.mv(ddy), 0.0)
-        d = torch.where(ok, v / torch.where(den == 0, 1.0, den), 0.0)
-        self.cs.append(c)
-        self.ds.append(d)
+        self.C[:, self.k] = torch.where(ok, ddx - self.mv(ddy), 0.0)
+        self.D[:, self.k] = torch.where(ok, v / torch.where(den == 0, 1.0, den), 0.0)
+        self.k += 1
 
 
 @dataclass
@@ -249,7 +275,7 @@ def run_scf(state: GeometryState) -> SCFResult:
     dev = state.dev
     x = torch.zeros((B, state.nvec), dtype=torch.float64, device=dev)
     y = x - state.g(x)
-    mix = BatchedBroyden(o.alpha, o.damp)
+    mix = BatchedBroyden(o.alpha, o.damp, B, state.nvec, o.maxiter + 1, dev)
     active = torch.ones(B, dtype=torch.bool, device=dev)
     iters = np.zeros(B, dtype=np.int64)
     conv = np.zeros(B, dtype=bool)

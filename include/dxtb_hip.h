@@ -177,6 +177,19 @@ int dxtb_forces_classical(const dxtb_basis_t* bas, const dxtb_params_t* par, con
                           const double* qdp, const double* qqp, double* dEdh, double* dEdcn, double* grad,
                           hipStream_t stream);
 
+/* Largest matrix order handled by dxtb_jacobi_eigh (LDS resident). */
+int dxtb_jacobi_max_n(void);
+
+/* Batched symmetric eigensolver, cyclic two-sided Jacobi, one workgroup per
+ * matrix.  A (nbatch, nmax, nmax): only the leading nsize[b] block of every
+ * matrix is used; V (nbatch, nmax, nmax) receives the eigenvectors (identity
+ * on the padding), evals (nbatch, nmax) the unsorted eigenvalues (0 on the
+ * padding).  Stops after max_sweeps or when max|offdiag| <= tol * max(1, max|diag|);
+ * sweeps (nbatch) may be NULL.  Replaces the per-iteration eigensolve of
+ * scf/pure/ovlp_diag.py:36-62 (xitorch exacteig, torch.linalg.eigh) for nmax <= 128. */
+int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, const double* A, double* V, double* evals,
+                     int max_sweeps, double tol, int* sweeps, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

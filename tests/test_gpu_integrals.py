@@ -93,3 +93,35 @@ def test_autograd_function(cuda, geometries):
     s, d, q = oint.multipole_matrices(b, p0)
     (s.sum() + 0.3 * d.sum() - 0.2 * q.sum()).backward()
     torch.testing.assert_close(p.grad[0].cpu(), p0.grad, atol=1e-10, rtol=0)
+
+
+def test_driver_eval_matrix_raw(cuda, geometries):
+    """IntDriverHip.eval_matrix returns the reference's raw origin-centred
+    integrals (S, r0 dipole, 9-component r0r0 quadrupole)."""
+    from dxtb_amd.integral_driver import DIPOLE_COMPONENTS, QUADRUPOLE_COMPONENTS, IntDriverHip
+
+    m = geometries["SiH4"]
+    nums = torch.tensor(m["numbers"], device=cuda)
+    pos = torch.tensor(m["positions"], dtype=torch.float64, device=cuda)
+    drv = IntDriverHip(nums, "gfn2", device=cuda)
+    drv.setup(pos)
+    d = drv.eval_matrix(DIPOLE_COMPONENTS).cpu()
+    q = drv.eval_matrix(QUADRUPOLE_COMPONENTS).cpu()
+    b = obasis.build(m["numbers"], "gfn2")
+    raw = oint.raw_matrices(b, torch.tensor(m["positions"], dtype=torch.float64))
+    torch.testing.assert_close(d, raw[1:4], atol=1e-10, rtol=0)
+    # raw unique comps xx, xy, xz, yy, yz, zz -> 9 row-major
+    idx = [4, 5, 6, 5, 7, 8, 6, 8, 9]
+    torch.testing.assert_close(q, raw[idx], atol=1e-10, rtol=0)
+
+
+def test_integrals_container_and_wrappers(cuda, geometries):
+    from dxtb_amd.integral_driver import dipint, overlap, quadint
+
+    m = geometries["H2O"]
+    nums = torch.tensor(m["numbers"], device=cuda)
+    pos = torch.tensor(m["positions"], dtype=torch.float64, device=cuda)
+    s, d, q = oint.multipole_matrices(obasis.build(m["numbers"], "gfn2"), torch.tensor(m["positions"], dtype=torch.float64))
+    torch.testing.assert_close(overlap(nums, pos, "gfn2").cpu(), s, atol=TOL, rtol=0)
+    torch.testing.assert_close(dipint(nums, pos, "gfn2").cpu(), d, atol=TOL, rtol=0)
+    torch.testing.assert_close(quadint(nums, pos, "gfn2").cpu(), q, atol=TOL, rtol=0)
