@@ -8,7 +8,7 @@ import of any compute function fails loudly: there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
-from ctypes import POINTER, c_double, c_int, c_void_p
+from ctypes import POINTER, c_double, c_int, c_size_t, c_void_p
 from pathlib import Path
 
 import torch
@@ -82,7 +82,9 @@ _SIGNATURES: dict[str, list] = {
     "dxtb_fermi": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p],
     "dxtb_forces_multipole": [POINTER(DxtbBasis)] + [c_void_p] * 9,
     "dxtb_jacobi_max_n": [],
-    "dxtb_jacobi_eigh": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p],
+    "dxtb_jacobi_work_size": [c_int, c_int],
+    "dxtb_jacobi_eigh": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double,
+                         c_void_p, c_void_p],
     "dxtb_forces_classical": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 19,
 }
 
@@ -106,9 +108,12 @@ def load() -> ctypes.CDLL:
         for name, args in _SIGNATURES.items():
             fn = getattr(lib, name)
             fn.argtypes = args
-            fn.restype = c_int
+            fn.restype = _RESTYPES.get(name, c_int)
         _lib = lib
     return _lib
+
+
+_RESTYPES = {"dxtb_jacobi_work_size": c_size_t}
 
 
 def check(rc: int, what: str) -> None:

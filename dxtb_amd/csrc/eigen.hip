@@ -1,25 +1,54 @@
-// Batched symmetric eigensolver for small matrices: two-sided cyclic Jacobi
-// with the round-robin (circle) parallel ordering, one workgroup per matrix,
-// the matrix resident in LDS (n <= 128, 132 KB with a padded stride), the
-// rotation accumulator V in global memory (L2 resident).
+// Batched symmetric eigensolvers for the SCF: two-sided cyclic Jacobi, one
+// workgroup per matrix, in two variants.
 //
-// In the SCF the solver is warm-started: the host passes F'' = C^T F' C with
-// C the previous iteration's eigenvectors, which is nearly diagonal, so a
-// couple of sweeps reach machine precision (Jacobi converges quadratically).
+// Scalar variant (n <= 128, A in LDS).  Every step of a sweep rotates n/2
+// disjoint index pairs chosen by the round-robin (circle) ordering.  The whole
+// similarity update A <- J^T A J of a step is one pass over the matrix: the
+// work item (k, k') owns the 2x2 block (rows of pair k, columns of pair k')
+// and applies the left rotation of pair k and the right rotation of pair k'
+// to it.  The eigenvectors are accumulated transposed (VT = V^T, one
+// eigenvector per row) in global memory: V <- V J rotates two rows of VT.
 //
-// Rotation (Numerical Recipes convention): for the pair (p, q)
+// Blocked variant (any n up to JMAX, A in global memory).  The matrix is cut
+// into blocks of 16 rows; the circle ordering runs over blocks, so each outer
+// step works on nb/2 disjoint block pairs (I, J).  For each pair the 32x32
+// subproblem A[IJ, IJ] is staged in LDS and one cyclic Jacobi sweep is run on
+// it by one wavefront (all pairs at the first step of an outer sweep, where
+// every block has a partner; the pairs across I and J afterwards, so each
+// index pair is rotated once per outer sweep), accumulating its 32x32
+// rotation Q.  The rest
+// of the matrix is then updated with dense products on the FP64 matrix cores
+// (v_mfma_f64_16x16x4): every off-diagonal 32x32 tile becomes
+// Q_k^T A[IJ_k, IJ_k'] Q_k' and every 32-column panel of VT rows IJ_k becomes
+// Q_k^T VT[IJ_k, :].  An outer sweep thus reads and writes A and VT about
+// nb times instead of n times (scalar Jacobi in global memory), and the
+// flops run at matrix-core rate.  This is a cyclic Jacobi method in a blocked
+// ordering; it converges like the scalar method.
+//
+// Warm start: in the SCF the host passes A = C^T F C with C the previous
+// iteration's orbitals, which is nearly diagonal; Jacobi converges
+// quadratically, so a few sweeps reach machine precision.
+//
+// Rotation (Numerical Recipes convention) for the pair (p, q):
 //   tau = (a_qq - a_pp) / (2 a_pq),  t = sign(tau) / (|tau| + sqrt(1 + tau^2)),
-//   c = 1/sqrt(1 + t^2), s = t c;  A <- J^T A J, V <- V J with
-//   J_pp = J_qq = c, J_pq = s, J_qp = -s.
+//   c = 1/sqrt(1 + t^2), s = t c;  J_pp = J_qq = c, J_pq = s, J_qp = -s.
 
 #include "common.h"
 #include "../../include/dxtb_hip.h"
 
 namespace {
 
-constexpr int JB = 256;          // threads per workgroup
-constexpr int JMAX = 128;        // largest matrix held in LDS
-constexpr int JSTRIDE = JMAX + 1;  // padded LDS row stride (doubles)
+constexpr int JB = 256;                // threads per workgroup (both variants)
+constexpr int NWAVE = JB / 64;         // wavefronts per workgroup
+constexpr int JMAX_LDS = 128;          // largest matrix of the scalar variant
+constexpr int JSTRIDE = JMAX_LDS + 1;  // padded LDS row stride (doubles)
+constexpr int JMAX = 512;              // largest matrix overall
+constexpr int JSCALAR_AUTO = 64;       // auto: scalar variant up to this order
+constexpr int BS = 16;                 // block size of the blocked variant
+constexpr int SB = 2 * BS;             // subproblem order
+constexpr int SST = SB + 1;            // LDS stride of a subproblem
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double block_max(double v, double* sh) {
 #pragma unroll
@@ -30,34 +59,70 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
   __syncthreads();
   double r = sh[0];
 #pragma unroll
-  for (int k = 1; k < JB / 64; ++k) r = fmax(r, sh[k]);
+  for (int k = 1; k < NWAVE; ++k) r = fmax(r, sh[k]);
   return r;
 }
 
-// slot -> player of the circle method at step s for m players (m even)
+// Orders LDS accesses of the lanes of one wavefront (lanes run in lockstep;
+// the fences keep the compiler from moving LDS accesses across this point).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// player at a slot of the circle method at step s for m players (m even)
 __device__ __forceinline__ int player(int slot, int s, int m) {
   return slot == 0 ? 0 : ((slot - 1 + s) % (m - 1)) + 1;
 }
 
-__global__ __launch_bounds__(JB) void k_jacobi(int nmax, const int* __restrict__ nsize, const double* __restrict__ Ain,
-                                               double* __restrict__ V, double* __restrict__ evals, int max_sweeps,
-                                               double tol, int* __restrict__ sweeps_out) {
-  extern __shared__ double lds[];
-  double* A = lds;                          // n x JSTRIDE
-  double* rc = lds + JMAX * JSTRIDE;        // cosines  (JMAX/2)
-  double* rs = rc + JMAX / 2;               // sines    (JMAX/2)
-  int* rp = (int*)(rs + JMAX / 2);          // p index  (JMAX/2)
-  int* rq = rp + JMAX / 2;                  // q index  (JMAX/2)
-  double* red = (double*)(rq + JMAX / 2);   // reduction scratch (JB/64)
+__device__ __forceinline__ bool rotation(double app, double aqq, double apq, double thr, double& c, double& sn) {
+  if (!(fabs(apq) > thr)) {
+    c = 1.0;
+    sn = 0.0;
+    return false;
+  }
+  // t = sign(tau) / (|tau| + sqrt(1 + tau^2)) with numerator and denominator
+  // multiplied by 2|a_pq|: one square root and one division
+  const double d = aqq - app;
+  const double t = (d >= 0.0 ? 2.0 * apq : -2.0 * apq) / (fabs(d) + sqrt(d * d + 4.0 * apq * apq));
+  c = rsqrt(1.0 + t * t);
+  sn = t * c;
+  return true;
+}
 
+// Stopping rule of both variants: the largest off-diagonal element is below
+// tol * max(1, max|a_ii|) (scale taken from the current diagonal, which tends
+// to the eigenvalues), or it has stopped shrinking within a factor 1000 of
+// that bar: Jacobi converges quadratically, so a sweep that removes less than
+// half of it has reached the rounding floor of the matrix.
+__device__ __forceinline__ bool stop_sweeps(double off, double off_prev, double thr, int sweep) {
+  if (off <= thr) return true;
+  return sweep > 0 && off > 0.5 * off_prev && off <= 1e3 * thr;
+}
+
+// ---------------------------------------------------------------- scalar ----
+
+__global__ __launch_bounds__(JB) void k_jacobi_scalar(int nmax, const int* __restrict__ nsize,
+                                                      const double* __restrict__ Ag, double* __restrict__ VT,
+                                                      double* __restrict__ evals, int max_sweeps, double tol,
+                                                      int* __restrict__ sweeps_out) {
+  extern __shared__ double lds[];
+  constexpr int NP = JMAX_LDS / 2;
   const int b = blockIdx.x;
-  const int n = nsize[b];
+  const int n = min(max(nsize[b], 0), min(nmax, JMAX_LDS));
   const size_t nn = (size_t)nmax * nmax;
-  const double* Ab = Ain + (size_t)b * nn;
-  double* Vb = V + (size_t)b * nn;
+  const double* Ab = Ag + (size_t)b * nn;
+  double* Vb = VT + (size_t)b * nn;
   double* eb = evals + (size_t)b * nmax;
 
-  // load the leading n x n block; V = I on the full padded block
+  double* A = lds;
+  double* rc = lds + JMAX_LDS * JSTRIDE;  // cosines (NP)
+  double* rs = rc + NP;                    // sines   (NP)
+  int* rp = (int*)(rs + NP);               // p index (NP)
+  int* rq = rp + NP;                       // q index (NP), -1 for the odd dummy
+  double* red = (double*)(rq + NP);
+
   for (int e = threadIdx.x; e < n * n; e += JB) {
     const int i = e / n, j = e - i * n;
     A[i * JSTRIDE + j] = Ab[(size_t)i * nmax + j];
@@ -68,21 +133,24 @@ __global__ __launch_bounds__(JB) void k_jacobi(int nmax, const int* __restrict__
   }
   __syncthreads();
 
-  const int m = n + (n & 1);  // even number of players, a dummy if n is odd
+  const int m = n + (n & 1);
   const int npair = m / 2;
   int sweep = 0;
-  // scale for the relative convergence test
-  double amax = 0.0;
-  for (int e = threadIdx.x; e < n; e += JB) amax = fmax(amax, fabs(A[e * JSTRIDE + e]));
-  amax = block_max(amax, red);
+  double off_prev = 0.0;
   for (; sweep < max_sweeps; ++sweep) {
-    double off = 0.0;
+    double off = 0.0, amax = 0.0;
     for (int e = threadIdx.x; e < n * n; e += JB) {
       const int i = e / n, j = e - i * n;
-      if (i != j) off = fmax(off, fabs(A[i * JSTRIDE + j]));
+      const double x = fabs(A[i * JSTRIDE + j]);
+      if (i != j) off = fmax(off, x);
+      else amax = fmax(amax, x);
     }
     off = block_max(off, red);
-    if (off <= tol * fmax(amax, 1.0)) break;
+    amax = block_max(amax, red);
+    const double thr_conv = tol * fmax(amax, 1.0);
+    if (stop_sweeps(off, off_prev, thr_conv, sweep)) break;
+    off_prev = off;
+    const double thr = fmax(1e-300, 1e-18 * fmax(amax, 1.0));
     for (int s = 0; s < m - 1; ++s) {
       for (int k = threadIdx.x; k < npair; k += JB) {
         int p = player(k, s, m), q = player(m - 1 - k, s, m);
@@ -92,52 +160,61 @@ __global__ __launch_bounds__(JB) void k_jacobi(int nmax, const int* __restrict__
           q = t;
         }
         double c = 1.0, sn = 0.0;
-        if (q < n) {
-          const double apq = A[p * JSTRIDE + q];
-          const double app = A[p * JSTRIDE + p], aqq = A[q * JSTRIDE + q];
-          if (fabs(apq) > 1e-300 && fabs(apq) > 1e-18 * (fabs(app) + fabs(aqq))) {
-            const double tau = (aqq - app) / (2.0 * apq);
-            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + t * t);
-            sn = t * c;
-          }
-        }
+        if (q < n) rotation(A[p * JSTRIDE + p], A[q * JSTRIDE + q], A[p * JSTRIDE + q], thr, c, sn);
         rc[k] = c;
         rs[k] = sn;
         rp[k] = p;
         rq[k] = (q < n) ? q : -1;
       }
       __syncthreads();
-      // columns of A and V: A <- A J, V <- V J
-      for (int e = threadIdx.x; e < npair * n; e += JB) {
-        const int k = e / n, i = e - k * n;
-        const int q = rq[k];
-        if (q < 0) continue;
-        const double sn = rs[k];
-        if (sn == 0.0) continue;
-        const int p = rp[k];
-        const double c = rc[k];
-        const double aip = A[i * JSTRIDE + p], aiq = A[i * JSTRIDE + q];
-        A[i * JSTRIDE + p] = c * aip - sn * aiq;
-        A[i * JSTRIDE + q] = sn * aip + c * aiq;
-        double* vi = Vb + (size_t)i * nmax;
-        const double vip = vi[p], viq = vi[q];
-        vi[p] = c * vip - sn * viq;
-        vi[q] = sn * vip + c * viq;
+      // A <- J^T A J, one 2x2 block per work item
+      for (int e = threadIdx.x; e < npair * npair; e += JB) {
+        const int k = e / npair, kk = e - k * npair;
+        const double c1 = rc[k], s1 = rs[k], c2 = rc[kk], s2 = rs[kk];
+        if (s1 == 0.0 && s2 == 0.0) continue;
+        const int p = rp[k], q = rq[k], pp = rp[kk], qq = rq[kk];
+        const double a = A[p * JSTRIDE + pp];
+        const double bq = (qq >= 0) ? A[p * JSTRIDE + qq] : 0.0;
+        const double cq = (q >= 0) ? A[q * JSTRIDE + pp] : 0.0;
+        const double d = (q >= 0 && qq >= 0) ? A[q * JSTRIDE + qq] : 0.0;
+        const double lpp = c1 * a - s1 * cq, lpq = c1 * bq - s1 * d;  // left (rows of pair k)
+        const double lqp = s1 * a + c1 * cq, lqq = s1 * bq + c1 * d;
+        A[p * JSTRIDE + pp] = c2 * lpp - s2 * lpq;  // right (columns of pair kk)
+        if (qq >= 0) A[p * JSTRIDE + qq] = s2 * lpp + c2 * lpq;
+        if (q >= 0) {
+          A[q * JSTRIDE + pp] = c2 * lqp - s2 * lqq;
+          if (qq >= 0) A[q * JSTRIDE + qq] = s2 * lqp + c2 * lqq;
+        }
       }
-      __syncthreads();
-      // rows of A: A <- J^T A
-      for (int e = threadIdx.x; e < npair * n; e += JB) {
-        const int k = e / n, j = e - k * n;
-        const int q = rq[k];
-        if (q < 0) continue;
-        const double sn = rs[k];
-        if (sn == 0.0) continue;
-        const int p = rp[k];
-        const double c = rc[k];
-        const double apj = A[p * JSTRIDE + j], aqj = A[q * JSTRIDE + j];
-        A[p * JSTRIDE + j] = c * apj - sn * aqj;
-        A[q * JSTRIDE + j] = sn * apj + c * aqj;
+      // VT <- J^T VT (rows p, q).  Four items per thread are loaded before any
+      // is stored (the rows of one step are disjoint), so the global-memory
+      // latency is paid once per four items.
+      const int nitem = npair * n;
+      for (int e0 = threadIdx.x; e0 < nitem; e0 += 4 * JB) {
+        double va[4], vb[4];
+        int op[4], oq[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int e = e0 + u * JB;
+          op[u] = -1;
+          if (e < nitem) {
+            const int k = e / n, i = e - k * n;
+            if (rq[k] >= 0 && rs[k] != 0.0) {
+              op[u] = rp[k] * nmax + i;
+              oq[u] = rq[k] * nmax + i;
+              va[u] = Vb[op[u]];
+              vb[u] = Vb[oq[u]];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (op[u] < 0) continue;
+          const int k = (e0 + u * JB) / n;
+          const double c = rc[k], sn = rs[k];
+          Vb[op[u]] = c * va[u] - sn * vb[u];
+          Vb[oq[u]] = sn * va[u] + c * vb[u];
+        }
       }
       __syncthreads();
     }
@@ -146,21 +223,406 @@ __global__ __launch_bounds__(JB) void k_jacobi(int nmax, const int* __restrict__
   if (threadIdx.x == 0 && sweeps_out != nullptr) sweeps_out[b] = sweep;
 }
 
+size_t scalar_shmem() {
+  constexpr size_t np = JMAX_LDS / 2;
+  return (size_t)JMAX_LDS * JSTRIDE * sizeof(double) + np * (2 * sizeof(double) + 2 * sizeof(int)) +
+         NWAVE * sizeof(double);
+}
+
+// --------------------------------------------------------------- blocked ----
+
+struct WaveLds {
+  double S[SB * SST];  // subproblem; reused as the staging tile of the products
+  double Q[SB * SST];  // accumulated rotation of the subproblem
+  double rc[BS], rs[BS];
+};
+
+constexpr int KMAX = JMAX / SB;  // max block pairs
+
+struct BlockedLds {
+  WaveLds w[NWAVE];
+  int rot[KMAX];  // pair k rotated anything in this outer step
+  double red[NWAVE];
+};
+
+// global row/column of local index r (0..31) of the block pair (bi, bj)
+__device__ __forceinline__ int gidx(int r, int bi, int bj) { return r < BS ? bi * BS + r : bj * BS + (r - BS); }
+
+// 1.0 or 0.0.  Out-of-range operands are loaded from a clamped (valid)
+// address and multiplied by this mask; unlike a select, the product keeps the
+// compiler from moving each load into a branch of its own, so all loads of a
+// batch are in flight together.  The loaded values are finite matrix entries.
+__device__ __forceinline__ double mask01(bool ok) { return ok ? 1.0 : 0.0; }
+
+// circle method for the 32 indices of a subproblem (slot < 32, step s < 31)
+__device__ __forceinline__ int player32(int slot, int s) {
+  if (slot == 0) return 0;
+  int t = slot - 1 + s;
+  if (t >= SB - 1) t -= SB - 1;
+  return t + 1;
+}
+// Pair j (< 16) of step s of a subproblem sweep.  Full sweep (31 steps): the
+// circle method over all 32 indices.  Cross sweep (16 steps): only pairs
+// between the two blocks, p = j, q = 16 + (j + s) mod 16.
+__device__ __forceinline__ void pair32(int j, int s, bool cross, int& p, int& q) {
+  if (cross) {
+    p = j;
+    q = BS + ((j + s) & (BS - 1));
+    return;
+  }
+  const int a = player32(j, s), b = player32(SB - 1 - j, s);
+  p = min(a, b);
+  q = max(a, b);
+}
+
+__device__ __forceinline__ d4_t mfma(double a, double b, d4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Row of element i of a lane in the accumulator of v_mfma_f64_16x16x4f64
+// (lane l, element i holds row l/16 + 4 i, column l%16 of the 16x16 block).
+__device__ __forceinline__ int acc_row(int lane, int i) { return (lane >> 4) + 4 * i; }
+
+// One cyclic Jacobi sweep on the 32x32 subproblem in W.S, accumulating its
+// rotations in W.Q (Q <- Q J): over all index pairs, or (cross) only over the
+// pairs between its two blocks.  Executed by one wavefront; returns whether
+// any rotation was applied (wave-uniform).  Each step is straight-line code:
+// every lane computes the rotation of its column pair itself (4 lanes per
+// pair, identical results), reads the other rotations it needs from LDS, then
+// loads all of its S and Q elements before it writes any of them (the 2x2
+// tiles and the Q row pairs of one step are disjoint).
+__device__ bool inner_sweep(WaveLds& W, double thr, bool cross, int lane) {
+  bool any = false;
+  const int kk = lane & 15;                    // column pair of this lane's S tiles
+  const int k0 = lane >> 4;                    // row pairs k0 + 4u, u < 4
+  const int kq0 = lane >> 5, iq = lane & 31;   // Q items: pairs kq0 + 2u (u < 8), row iq
+  const int nsteps = cross ? BS : SB - 1;
+  for (int s = 0; s < nsteps; ++s) {
+    int pp, qq;
+    pair32(kk, s, cross, pp, qq);
+    double c2, s2;
+    any |= rotation(W.S[pp * SST + pp], W.S[qq * SST + qq], W.S[pp * SST + qq], thr, c2, s2);
+    if (lane < BS) {
+      W.rc[lane] = c2;
+      W.rs[lane] = s2;
+    }
+    wave_sync();
+    double c1[4], s1[4], x[4][4];
+    int p1[4], q1[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + 4 * u;
+      c1[u] = W.rc[k];
+      s1[u] = W.rs[k];
+      pair32(k, s, cross, p1[u], q1[u]);
+      x[u][0] = W.S[p1[u] * SST + pp];
+      x[u][1] = W.S[p1[u] * SST + qq];
+      x[u][2] = W.S[q1[u] * SST + pp];
+      x[u][3] = W.S[q1[u] * SST + qq];
+    }
+    double cq[8], sq[8], y[8][2];
+    int pq[8], qv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = kq0 + 2 * u;
+      cq[u] = W.rc[k];
+      sq[u] = W.rs[k];
+      pair32(k, s, cross, pq[u], qv[u]);
+      y[u][0] = W.Q[iq * SST + pq[u]];
+      y[u][1] = W.Q[iq * SST + qv[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {  // S <- J^T S J on the 2x2 tile (row pair k, column pair kk)
+      const double lpp = c1[u] * x[u][0] - s1[u] * x[u][2], lpq = c1[u] * x[u][1] - s1[u] * x[u][3];
+      const double lqp = s1[u] * x[u][0] + c1[u] * x[u][2], lqq = s1[u] * x[u][1] + c1[u] * x[u][3];
+      W.S[p1[u] * SST + pp] = c2 * lpp - s2 * lpq;
+      W.S[p1[u] * SST + qq] = s2 * lpp + c2 * lpq;
+      W.S[q1[u] * SST + pp] = c2 * lqp - s2 * lqq;
+      W.S[q1[u] * SST + qq] = s2 * lqp + c2 * lqq;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {  // Q <- Q J: columns p, q of row iq
+      W.Q[iq * SST + pq[u]] = cq[u] * y[u][0] - sq[u] * y[u][1];
+      W.Q[iq * SST + qv[u]] = sq[u] * y[u][0] + cq[u] * y[u][1];
+    }
+    wave_sync();
+  }
+  return __any(any ? 1 : 0) != 0;
+}
+
+// out(32x32) = Q^T X for the 32x32 operand X given by a loader; Q is the
+// 32x32 rotation at Qg (row-major).  acc[rb][cb] holds the 16x16 block
+// (rb, cb) in the accumulator layout of acc_row.  All operand fragments are
+// loaded before the first product so that their latencies overlap.
+template <class Load>
+__device__ __forceinline__ void qt_times(const double* __restrict__ Qg, Load load, int lane, d4_t acc[2][2]) {
+  const int lr = lane & 15, lk = lane >> 4;
+  double a0[8], a1[8], b0[8], b1[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int m = 4 * t + lk;
+    // A operand (Q^T)[r][m] = Q[m][r]: lane l supplies row l%16, column l/16
+    a0[t] = Qg[m * SB + lr];
+    a1[t] = Qg[m * SB + BS + lr];
+    // B operand X[m][c]: lane l supplies row l/16, column l%16
+    b0[t] = load(m, lr);
+    b1[t] = load(m, BS + lr);
+  }
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = d4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    acc[0][0] = mfma(a0[t], b0[t], acc[0][0]);
+    acc[0][1] = mfma(a0[t], b1[t], acc[0][1]);
+    acc[1][0] = mfma(a1[t], b0[t], acc[1][0]);
+    acc[1][1] = mfma(a1[t], b1[t], acc[1][1]);
+  }
+}
+
+// block pair k of outer step s over nb blocks, bi < bj
+__device__ __forceinline__ void block_pair(int k, int s, int nb, int& bi, int& bj) {
+  const int a = player(k, s, nb), c = player(nb - 1 - k, s, nb);
+  bi = min(a, c);
+  bj = max(a, c);
+}
+
+__global__ __launch_bounds__(JB, 2) void k_jacobi_blocked(int nmax, const int* __restrict__ nsize, double* __restrict__ Ag,
+                                                       double* __restrict__ VT, double* __restrict__ evals,
+                                                       double* __restrict__ work, int max_sweeps, double tol,
+                                                       int* __restrict__ sweeps_out) {
+  extern __shared__ double lds_raw[];
+  BlockedLds& L = *reinterpret_cast<BlockedLds*>(lds_raw);
+  const int b = blockIdx.x;
+  const int n = min(max(nsize[b], 0), nmax);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int npad = (nmax + SB - 1) / SB * SB;
+  double* A = Ag + (size_t)b * nmax * nmax;
+  double* Vb = VT + (size_t)b * nmax * nmax;
+  double* eb = evals + (size_t)b * nmax;
+  double* Qw = work + (size_t)b * npad * SB;  // K rotations of 32x32
+
+  for (int e = threadIdx.x; e < nmax * nmax; e += JB) {
+    const int i = e / nmax, j = e - i * nmax;
+    Vb[e] = (i == j) ? 1.0 : 0.0;
+  }
+  if (n == 0) {
+    for (int i = threadIdx.x; i < nmax; i += JB) eb[i] = 0.0;
+    if (threadIdx.x == 0 && sweeps_out != nullptr) sweeps_out[b] = 0;
+    return;
+  }
+  const int npairs = (n + SB - 1) / SB;  // block pairs (n padded to 32)
+  const int nb = 2 * npairs;             // blocks (even)
+  const int nlast = n - 1;               // clamp for branch-free loads
+
+  WaveLds& W = L.w[wv];
+
+  int sweep = 0;
+  double off_prev = 0.0;
+  for (; sweep < max_sweeps; ++sweep) {
+    // max |offdiag| and max |diag|, eight independent loads in flight per thread
+    double off = 0.0, amax = 0.0;
+    const int nn = n * n;
+    for (int e0 = threadIdx.x; e0 < nn; e0 += 8 * JB) {
+      double v[8];
+      bool dg[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = min(e0 + u * JB, nn - 1);
+        const int i = e / n, j = e - i * n;
+        const double x = A[(size_t)i * nmax + j];
+        v[u] = fabs(x) * mask01(e0 + u * JB < nn);
+        dg[u] = i == j;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        off = fmax(off, dg[u] ? 0.0 : v[u]);
+        amax = fmax(amax, dg[u] ? v[u] : 0.0);
+      }
+    }
+    off = block_max(off, L.red);  // ends with a barrier: V init and last step visible
+    amax = block_max(amax, L.red);
+    const double thr = fmax(1e-300, tol * fmax(amax, 1.0));
+    if (stop_sweeps(off, off_prev, thr, sweep)) break;
+    off_prev = off;
+    for (int s = 0; s < nb - 1; ++s) {
+      // ---- inner phase: one wavefront per block pair
+      for (int k = wv; k < npairs; k += NWAVE) {
+        int bi, bj;
+        block_pair(k, s, nb, bi, bj);
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int e = lane + 64 * u;
+          const int gr = gidx(e >> 5, bi, bj), gc = gidx(e & 31, bi, bj);
+          const double x = A[(size_t)min(gr, nlast) * nmax + min(gc, nlast)];
+          v[u] = x * mask01(gr < n && gc < n);
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int e = lane + 64 * u;
+          const int r = e >> 5, c = e & 31;
+          W.S[r * SST + c] = v[u];
+          W.Q[r * SST + c] = (r == c) ? 1.0 : 0.0;
+        }
+        wave_sync();
+        // every block meets a partner at step 0: the pairs inside the blocks
+        // are rotated there, later steps rotate only the pairs across blocks
+        const bool any = inner_sweep(W, thr, s > 0, lane);
+        if (lane == 0) L.rot[k] = any ? 1 : 0;
+        if (any) {
+#pragma unroll 4
+          for (int u = 0; u < 16; ++u) {
+            const int e = lane + 64 * u;
+            const int r = e >> 5, c = e & 31;
+            const int gr = gidx(r, bi, bj), gc = gidx(c, bi, bj);
+            if (gr < n && gc < n) A[(size_t)gr * nmax + gc] = W.S[r * SST + c];
+            Qw[(size_t)k * SB * SB + e] = W.Q[r * SST + c];
+          }
+        }
+        wave_sync();
+      }
+      __syncthreads();
+      // ---- product phase: off-diagonal tiles (k < k') and VT panels
+      const int ntile = npairs * (npairs - 1) / 2;
+      const int npanel = (n + SB - 1) / SB;
+      const int nwork = ntile + npairs * npanel;
+      for (int item = wv; item < nwork; item += NWAVE) {
+        if (item < ntile) {
+          int k = 0, rem = item;  // tile index -> (k, k2), k < k2
+          while (rem >= npairs - 1 - k) {
+            rem -= npairs - 1 - k;
+            ++k;
+          }
+          const int k2 = k + 1 + rem;
+          const bool rk = L.rot[k] != 0, rk2 = L.rot[k2] != 0;
+          if (!rk && !rk2) continue;
+          int bi, bj, ci, cj;
+          block_pair(k, s, nb, bi, bj);
+          block_pair(k2, s, nb, ci, cj);
+          auto loadA = [&](int m, int c) -> double {
+            const int gr = gidx(m, bi, bj), gc = gidx(c, ci, cj);
+            const double x = A[(size_t)min(gr, nlast) * nmax + min(gc, nlast)];
+            return x * mask01(gr < n && gc < n);
+          };
+          d4_t acc[2][2];
+          if (rk) {
+            qt_times(Qw + (size_t)k * SB * SB, loadA, lane, acc);
+          } else {  // Q_k = 1: T is the tile itself
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+              for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[rb][cb][i] = loadA(rb * BS + acc_row(lane, i), cb * BS + (lane & 15));
+          }
+          // T = Q_k^T X -> LDS (row-major, stride SST)
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+              for (int i = 0; i < 4; ++i)
+                W.S[(rb * BS + acc_row(lane, i)) * SST + cb * BS + (lane & 15)] = acc[rb][cb][i];
+          wave_sync();
+          if (rk2) {
+            // U^T = Q_k2^T T^T with U = T Q_k2: the same product, T read transposed
+            auto loadT = [&](int m, int c) -> double { return W.S[c * SST + m]; };
+            qt_times(Qw + (size_t)k2 * SB * SB, loadT, lane, acc);
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+              for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                  const int uc = rb * BS + acc_row(lane, i), ur = cb * BS + (lane & 15);
+                  const int gr = gidx(ur, bi, bj), gc = gidx(uc, ci, cj);
+                  if (gr < n && gc < n) {
+                    const double v = acc[rb][cb][i];
+                    A[(size_t)gr * nmax + gc] = v;
+                    A[(size_t)gc * nmax + gr] = v;
+                  }
+                }
+          } else {  // Q_k2 = 1: U = T
+            for (int e = lane; e < SB * SB; e += 64) {
+              const int r = e >> 5, c = e & 31;
+              const int gr = gidx(r, bi, bj), gc = gidx(c, ci, cj);
+              if (gr < n && gc < n) {
+                const double v = W.S[r * SST + c];
+                A[(size_t)gr * nmax + gc] = v;
+                A[(size_t)gc * nmax + gr] = v;
+              }
+            }
+          }
+          wave_sync();
+        } else {
+          const int w2 = item - ntile;
+          const int k = w2 / npanel, pc = w2 - k * npanel;
+          if (!L.rot[k]) continue;
+          int bi, bj;
+          block_pair(k, s, nb, bi, bj);
+          const int c0 = pc * SB;
+          auto loadV = [&](int m, int c) -> double {
+            const int gr = gidx(m, bi, bj), gc = c0 + c;
+            const double x = Vb[(size_t)min(gr, nlast) * nmax + min(gc, nlast)];
+            return x * mask01(gr < n && gc < n);
+          };
+          d4_t acc[2][2];
+          qt_times(Qw + (size_t)k * SB * SB, loadV, lane, acc);
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                const int gr = gidx(rb * BS + acc_row(lane, i), bi, bj), gc = c0 + cb * BS + (lane & 15);
+                if (gr < n && gc < n) Vb[(size_t)gr * nmax + gc] = acc[rb][cb][i];
+              }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < nmax; i += JB) eb[i] = (i < n) ? A[(size_t)i * nmax + i] : 0.0;
+  if (threadIdx.x == 0 && sweeps_out != nullptr) sweeps_out[b] = sweep;
+}
+
+bool attr_done[64];
+
 }  // namespace
 
 extern "C" int dxtb_jacobi_max_n(void) { return JMAX; }
 
-extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, const double* A, double* V, double* evals,
-                                int max_sweeps, double tol, int* sweeps, hipStream_t st) {
-  if (nbatch <= 0 || nmax <= 0 || nmax > JMAX || !nsize || !A || !V || !evals || max_sweeps < 0) return DXTB_EINVAL;
-  const size_t shmem = (size_t)JMAX * JSTRIDE * sizeof(double) + JMAX * sizeof(double) +
-                       JMAX * sizeof(int) + (JB / 64) * sizeof(double);
-  static bool attr_set = false;
-  if (!attr_set) {
-    DXTB_CHECK(hipFuncSetAttribute((const void*)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    attr_set = true;
+extern "C" size_t dxtb_jacobi_work_size(int nbatch, int nmax) {
+  if (nbatch <= 0 || nmax <= 0) return 0;
+  const size_t npad = (size_t)(nmax + SB - 1) / SB * SB;
+  return (size_t)nbatch * npad * SB * sizeof(double);
+}
+
+extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals,
+                                double* work, int variant, int max_sweeps, double tol, int* sweeps, hipStream_t st) {
+  if (nbatch <= 0 || nmax <= 0 || nmax > JMAX || !nsize || !A || !VT || !evals || max_sweeps < 0 || variant < 0 ||
+      variant > 2)
+    return DXTB_EINVAL;
+  const bool scalar = variant == 1 || (variant == 0 && nmax <= JSCALAR_AUTO);
+  if (scalar && nmax > JMAX_LDS) return DXTB_EINVAL;
+  if (!scalar && !work) return DXTB_EINVAL;
+  int dev = 0;
+  DXTB_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return DXTB_EINVAL;
+  if (!attr_done[dev]) {
+    DXTB_CHECK(hipFuncSetAttribute((const void*)k_jacobi_scalar, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)scalar_shmem()));
+    DXTB_CHECK(hipFuncSetAttribute((const void*)k_jacobi_blocked, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)sizeof(BlockedLds)));
+    attr_done[dev] = true;
   }
-  hipLaunchKernelGGL(k_jacobi, dim3((unsigned)nbatch), dim3(JB), shmem, st, nmax, nsize, A, V, evals, max_sweeps, tol,
-                     sweeps);
+  if (scalar)
+    hipLaunchKernelGGL(k_jacobi_scalar, dim3((unsigned)nbatch), dim3(JB), scalar_shmem(), st, nmax, nsize, A, VT,
+                       evals, max_sweeps, tol, sweeps);
+  else
+    hipLaunchKernelGGL(k_jacobi_blocked, dim3((unsigned)nbatch), dim3(JB), sizeof(BlockedLds), st, nmax, nsize, A,
+                       VT, evals, work, max_sweeps, tol, sweeps);
   return (int)hipGetLastError();
 }

@@ -35,6 +35,10 @@ from .integrals import multipole_forward
 from .params_dev import device_params
 
 PAD_DIAG = 1.0e3  # Fock diagonal of padding AOs: above every real level
+# Jacobi stops when max|offdiag| <= JACOBI_TOL * max(1, max|diag|): orbital
+# errors are then O(1e-12 Eh / gap), far below what the 1e-8 Eh energy and
+# 1e-6 Eh/Bohr force bars can see (the energy is variational in the orbitals).
+JACOBI_TOL = 1.0e-12
 
 
 @dataclass
@@ -45,7 +49,7 @@ class SCFOptions:
     maxiter: int = 250
     damp: float = 0.5
     alpha: float = -0.5
-    eigensolver: str = "jacobi"  # "jacobi": hand-written LDS Jacobi for nao <= 128, else device eigh
+    eigensolver: str = "jacobi"  # "jacobi": hand-written Jacobi for nao <= 512, else device eigh
 
 
 class GeometryState:
@@ -107,6 +111,35 @@ class GeometryState:
         self.nvec = ns + (9 * na if self.mp else 0)
         self.valid_sh = (torch.arange(ns, device=dev)[None, :] < d["nsh"][:, None]).double()
         self.C = None  # previous orbitals (warm start of the Jacobi solver)
+        self._jwork = None  # Jacobi workspace (a subset keeps the larger buffer)
+
+    # ---- culling: a state restricted to some molecules of the batch --------
+    _PER_MOLECULE = ("pos", "cn", "fsh", "dfsh", "hfsh", "erep", "mrad", "S", "D", "Q", "H0", "X", "nab", "n0",
+                     "valid_sh", "C")
+    _GLOBAL_DEV = ("ush_l", "ush_np", "ush_alpha", "ush_coeff", "item_mol", "item_pair")
+
+    def subset(self, keep: torch.Tensor) -> "GeometryState":
+        """Shallow copy holding only the molecules ``keep`` (culling of
+        converged systems, as the reference's batched SCF does in
+        scf/unrolling/default.py:241-322).  The integral work lists are not
+        carried over (they are not used by the SCF iteration)."""
+        import copy
+
+        new = copy.copy(self)
+        kc = keep.cpu().numpy()
+        for name in self._PER_MOLECULE:
+            t = getattr(self, name)
+            if t is not None:
+                setattr(new, name, t.index_select(0, keep).contiguous())
+        bb = self.bb
+        dev = {k: (v if k in self._GLOBAL_DEV else v.index_select(0, keep).contiguous()) for k, v in bb.dev.items()}
+        new.bb = BatchBasis(
+            bb.gfn, bb.numbers[kc], bb.nat[kc], bb.nsh[kc], bb.nao[kc], bb.nat_max, bb.nsh_max, bb.nao_max,
+            bb.sh_atom[kc], bb.sh_ush[kc], bb.sh_ao[kc], bb.sh_l[kc], bb.ao_atom[kc], bb.ao_shell[kc],
+            bb.item_mol[:0], bb.item_pair[:0], np.zeros(10, dtype=np.int32), dev,
+        )
+        new.bs = _lib.basis_struct(new.bb)
+        return new
 
     # ---- one application of the SCF map -----------------------------------
     def unpack(self, x):
@@ -133,21 +166,25 @@ class GeometryState:
     def solve(self, F):
         """Orbitals, occupations and density of a Fock matrix.
 
-        Small bases: the hand-written LDS Jacobi solver on A = C^T F C with C
+        nao <= 512: the hand-written Jacobi solver on A = C^T F C with C
         the previous iteration's orbitals (S-orthonormal, so A is the Fock
         matrix in the old orbital basis and nearly diagonal; the first
-        iteration uses C = X = L^-T); the new orbitals are C V.  Larger bases:
-        X^T F X and the device eigh (rocSOLVER)."""
+        iteration uses C = X = L^-T); the new orbitals are C V.  Larger bases
+        (and eigensolver="device"): X^T F X and the device eigh (rocSOLVER)."""
         n = self.bb.nao_max
         if self.opts.eigensolver == "jacobi" and n <= self.lib.dxtb_jacobi_max_n():
             Cb = self.X if self.C is None else self.C
             A = (Cb.transpose(-1, -2) @ F @ Cb).contiguous()
             V = torch.empty_like(A)
             e = torch.empty((self.bb.nbatch, n), dtype=torch.float64, device=self.dev)
+            if self._jwork is None:
+                nbytes = self.lib.dxtb_jacobi_work_size(self.bb.nbatch, n)
+                self._jwork = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=self.dev)
             rc = self.lib.dxtb_jacobi_eigh(self.bb.nbatch, n, _lib.ptr(self.bb.dev["nao"]), _lib.ptr(A),
-                                           _lib.ptr(V), _lib.ptr(e), 50, 1e-14, None, _lib.stream_ptr(self.dev))
+                                           _lib.ptr(V), _lib.ptr(e), _lib.ptr(self._jwork), 0, 50, JACOBI_TOL, None,
+                                           _lib.stream_ptr(self.dev))
             _lib.check(rc, "dxtb_jacobi_eigh")
-            C = Cb @ V
+            C = torch.bmm(Cb, V.transpose(1, 2).contiguous())  # V holds the eigenvectors as rows
             self.C = C
         else:
             Fp = self.X.transpose(-1, -2) @ F @ self.X
@@ -269,12 +306,18 @@ class SCFResult:
     state: GeometryState
 
 
-def run_scf(state: GeometryState) -> SCFResult:
+def run_scf(state: GeometryState, cull: float = 0.75) -> SCFResult:
+    """Iterate all molecules of ``state`` to self-consistency.  When the
+    active molecules drop below ``cull`` of the current batch, the batch is
+    compacted to the active ones (converged molecules are frozen)."""
     o = state.opts
     B = state.bb.nbatch
     dev = state.dev
-    x = torch.zeros((B, state.nvec), dtype=torch.float64, device=dev)
-    y = x - state.g(x)
+    x_full = torch.zeros((B, state.nvec), dtype=torch.float64, device=dev)
+    idx = torch.arange(B, device=dev)  # full-batch index of every current row
+    cur = state
+    x = x_full.clone()
+    y = x - cur.g(x)
     mix = BatchedBroyden(o.alpha, o.damp, B, state.nvec, o.maxiter + 1, dev)
     active = torch.ones(B, dtype=torch.bool, device=dev)
     iters = np.zeros(B, dtype=np.int64)
@@ -284,20 +327,34 @@ def run_scf(state: GeometryState) -> SCFResult:
         dx = -mix.mv(y)
         dxn = dx.norm(dim=-1)
         xnew = torch.where(active.unsqueeze(-1), x + o.damp * dx, x)
-        ynew = xnew - state.g(xnew)
+        ynew = xnew - cur.g(xnew)
         ynew = torch.where(active.unsqueeze(-1), ynew, y)
         yn = ynew.norm(dim=-1)
         mix.update(xnew - xprev, ynew - yprev, active)
         xprev, yprev = xnew, ynew
         done = active & (dxn < o.x_tol) & (yn < o.f_tol)
-        d_host = done.cpu().numpy()
-        a_host = active.cpu().numpy()
-        iters[a_host] = it + 1
-        conv |= d_host
-        active = active & ~done
         x, y = xnew, ynew
-        if not bool(active.any()):
+        a_host = active.cpu().numpy()
+        d_host = done.cpu().numpy()
+        rows = idx.cpu().numpy()
+        iters[rows[a_host]] = it + 1
+        conv[rows[d_host]] = True
+        active = active & ~done
+        if d_host.any():
+            x_full[idx[done]] = x[done]
+        n_act = int(active.sum())
+        if n_act == 0:
             break
+        if n_act < cull * cur.bb.nbatch:
+            keep = torch.nonzero(active).squeeze(-1)
+            cur = cur.subset(keep)
+            idx = idx[keep]
+            x, y, xprev, yprev = x[keep], y[keep], xprev[keep], yprev[keep]
+            active = active[keep]
+            mix.C, mix.D = mix.C[keep], mix.D[keep]
+    if bool(active.any()):
+        x_full[idx[active]] = x[active]
+    x = x_full
     # final state at the converged potential
     F = state.fock(x)
     P, e, C, occ, ts = state.solve(F)

@@ -12,13 +12,13 @@
  * The AO order is the reference's: atoms in input order, shells in parameter
  * order, m = -l..l per shell (p as y, z, x).
  *
- * The Python host (dxtb_amd) binds this header with ctypes; INTEGRATION.md
- * shows the binding.
+ * The Python host binds this header with ctypes (dxtb_amd/_lib.py).
  */
 #ifndef DXTB_HIP_H
 #define DXTB_HIP_H
 
 #include <hip/hip_runtime_api.h>
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -177,18 +177,27 @@ int dxtb_forces_classical(const dxtb_basis_t* bas, const dxtb_params_t* par, con
                           const double* qdp, const double* qqp, double* dEdh, double* dEdcn, double* grad,
                           hipStream_t stream);
 
-/* Largest matrix order handled by dxtb_jacobi_eigh (LDS resident). */
+/* Largest matrix order handled by dxtb_jacobi_eigh. */
 int dxtb_jacobi_max_n(void);
 
+/* Bytes of the workspace dxtb_jacobi_eigh needs for the blocked variant. */
+size_t dxtb_jacobi_work_size(int nbatch, int nmax);
+
 /* Batched symmetric eigensolver, cyclic two-sided Jacobi, one workgroup per
- * matrix.  A (nbatch, nmax, nmax): only the leading nsize[b] block of every
- * matrix is used; V (nbatch, nmax, nmax) receives the eigenvectors (identity
- * on the padding), evals (nbatch, nmax) the unsorted eigenvalues (0 on the
- * padding).  Stops after max_sweeps or when max|offdiag| <= tol * max(1, max|diag|);
+ * matrix.  variant 1: scalar Jacobi with A in LDS (nmax <= 128); variant 2:
+ * blocked Jacobi (16-row blocks, FP64 matrix-core updates, A in global
+ * memory); variant 0 picks by nmax.
+ * A (nbatch, nmax, nmax): only the leading nsize[b] block of every matrix is
+ * used; the blocked variant overwrites it.  VT (nbatch, nmax, nmax) receives
+ * the eigenvectors as ROWS (identity on the padding), evals (nbatch, nmax)
+ * the unsorted eigenvalues (0 on the padding).  work: device buffer of
+ * dxtb_jacobi_work_size bytes (may be NULL for variant 1).  Stops after
+ * max_sweeps, when max|offdiag| <= tol * max(1, max|diag|), or when a sweep
+ * no longer halves max|offdiag| within 1000x of that bar (rounding floor);
  * sweeps (nbatch) may be NULL.  Replaces the per-iteration eigensolve of
- * scf/pure/ovlp_diag.py:36-62 (xitorch exacteig, torch.linalg.eigh) for nmax <= 128. */
-int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, const double* A, double* V, double* evals,
-                     int max_sweeps, double tol, int* sweeps, hipStream_t stream);
+ * scf/pure/ovlp_diag.py:36-62 (xitorch exacteig / torch.linalg.eigh). */
+int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals, double* work,
+                     int variant, int max_sweeps, double tol, int* sweeps, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -15,7 +15,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def _declared() -> set[str]:
     text = "\n".join(p.read_text() for p in (ROOT / "include").glob("*.h"))
-    return set(re.findall(r"^\s*int\s+(dxtb_\w+)\s*\(", text, flags=re.M))
+    return set(re.findall(r"^\s*(?:int|size_t)\s+(dxtb_\w+)\s*\(", text, flags=re.M))
 
 
 def test_library_present():
@@ -43,3 +43,14 @@ def test_invalid_arguments_rejected_without_gpu():
     st = _lib.DxtbBasis()  # all zero: invalid
     assert lib.dxtb_multipole_fwd(st, None, None, None, None, None, None, 1, None) == _lib.DXTB_EINVAL
     assert lib.dxtb_multipole_vjp(st, None, None, None, None, None, None) == _lib.DXTB_EINVAL
+
+
+def test_jacobi_arguments_without_gpu():
+    lib = _lib.load()
+    # workspace: 32 rows of the order padded to a multiple of 32, per matrix
+    assert lib.dxtb_jacobi_work_size(2, 33) == 2 * 64 * 32 * 8
+    assert lib.dxtb_jacobi_work_size(0, 33) == 0
+    assert lib.dxtb_jacobi_max_n() >= 128
+    args = (None,) * 5
+    assert lib.dxtb_jacobi_eigh(0, 10, *args, 0, 10, 1e-14, None, None) == _lib.DXTB_EINVAL
+    assert lib.dxtb_jacobi_eigh(1, lib.dxtb_jacobi_max_n() + 1, *args, 0, 10, 1e-14, None, None) == _lib.DXTB_EINVAL
