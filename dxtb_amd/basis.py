@@ -147,9 +147,20 @@ class BatchBasis:
         return int(self.numbers.shape[0])
 
 
-def build_batch(numbers, gfn: str, device: torch.device | str) -> BatchBasis:
-    """Shell maps and ordered shell-pair lists for a (padded) batch."""
+def build_batch(numbers, gfn: str, device: torch.device | str, dims: tuple[int, int, int] | None = None
+                ) -> BatchBasis:
+    """Shell maps and ordered shell-pair lists for a (padded) batch.
+
+    ``dims = (nat_max, nsh_max, nao_max)`` pads the batch to fixed sizes (slot
+    pools of the serving engine hold molecules of different sizes in arrays
+    of one shape); by default the sizes are the batch maxima."""
     nums = np.atleast_2d(np.asarray(numbers.cpu() if torch.is_tensor(numbers) else numbers, dtype=np.int64))
+    if dims is not None:
+        if nums.shape[1] > dims[0]:
+            if (nums[:, dims[0]:] != 0).any():
+                raise ValueError(f"a molecule has more than {dims[0]} atoms")
+            nums = nums[:, : dims[0]]
+        nums = np.pad(nums, ((0, 0), (0, dims[0] - nums.shape[1])))
     table = shell_table(gfn)
     B, nat_max = nums.shape
     nat = (nums != 0).sum(1)
@@ -159,6 +170,10 @@ def build_batch(numbers, gfn: str, device: torch.device | str) -> BatchBasis:
         raise ValueError(f"Elements {bad} are not parametrized in {gfn}.")
     nsh = cnt.sum(1)
     nsh_max = int(max(nsh.max(), 1))
+    if dims is not None:
+        if nsh_max > dims[1]:
+            raise ValueError(f"a molecule has more than {dims[1]} shells")
+        nsh_max = int(dims[1])
     if nsh_max > 0xFFFF:
         raise ValueError("too many shells per molecule")
     sh_atom = np.zeros((B, nsh_max), dtype=np.int32)
@@ -180,6 +195,10 @@ def build_batch(numbers, gfn: str, device: torch.device | str) -> BatchBasis:
     sh_ao = (np.cumsum(nsph, 1) - nsph).astype(np.int32)
     nao = nsph.sum(1)
     nao_max = int(max(nao.max(), 1))
+    if dims is not None:
+        if nao_max > dims[2]:
+            raise ValueError(f"a molecule has more than {dims[2]} basis functions")
+        nao_max = int(dims[2])
     ao_atom = np.zeros((B, nao_max), dtype=np.int32)
     ao_shell = np.zeros((B, nao_max), dtype=np.int32)
     for b in range(B):

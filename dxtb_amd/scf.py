@@ -118,11 +118,15 @@ class GeometryState:
                      "valid_sh", "C")
     _GLOBAL_DEV = ("ush_l", "ush_np", "ush_alpha", "ush_coeff", "item_mol", "item_pair")
 
-    def subset(self, keep: torch.Tensor) -> "GeometryState":
-        """Shallow copy holding only the molecules ``keep`` (culling of
-        converged systems, as the reference's batched SCF does in
+    _BB_HOST = ("numbers", "nat", "nsh", "nao", "sh_atom", "sh_ush", "sh_ao", "sh_l", "ao_atom", "ao_shell")
+
+    def subset(self, keep: torch.Tensor, bb: BatchBasis | None = None) -> "GeometryState":
+        """Copy holding only the molecules ``keep`` (culling of converged
+        systems, as the reference's batched SCF does in
         scf/unrolling/default.py:241-322).  The integral work lists are not
-        carried over (they are not used by the SCF iteration)."""
+        carried over (the SCF iteration does not use them) unless ``bb``, the
+        basis of exactly these molecules built with the same padded sizes, is
+        given (the force kernels need them)."""
         import copy
 
         new = copy.copy(self)
@@ -131,15 +135,64 @@ class GeometryState:
             t = getattr(self, name)
             if t is not None:
                 setattr(new, name, t.index_select(0, keep).contiguous())
-        bb = self.bb
-        dev = {k: (v if k in self._GLOBAL_DEV else v.index_select(0, keep).contiguous()) for k, v in bb.dev.items()}
-        new.bb = BatchBasis(
-            bb.gfn, bb.numbers[kc], bb.nat[kc], bb.nsh[kc], bb.nao[kc], bb.nat_max, bb.nsh_max, bb.nao_max,
-            bb.sh_atom[kc], bb.sh_ush[kc], bb.sh_ao[kc], bb.sh_l[kc], bb.ao_atom[kc], bb.ao_shell[kc],
-            bb.item_mol[:0], bb.item_pair[:0], np.zeros(10, dtype=np.int32), dev,
-        )
+        if bb is None:
+            ob = self.bb
+            dev = {k: (v if k in self._GLOBAL_DEV else v.index_select(0, keep).contiguous())
+                   for k, v in ob.dev.items()}
+            bb = BatchBasis(
+                ob.gfn, ob.numbers[kc], ob.nat[kc], ob.nsh[kc], ob.nao[kc], ob.nat_max, ob.nsh_max, ob.nao_max,
+                ob.sh_atom[kc], ob.sh_ush[kc], ob.sh_ao[kc], ob.sh_l[kc], ob.ao_atom[kc], ob.ao_shell[kc],
+                ob.item_mol[:0], ob.item_pair[:0], np.zeros(10, dtype=np.int32), dev,
+            )
+        elif (bb.nat_max, bb.nsh_max, bb.nao_max) != (self.bb.nat_max, self.bb.nsh_max, self.bb.nao_max):
+            raise ValueError("subset basis must have the padded sizes of the state")
+        new.bb = bb
+        new.bs = _lib.basis_struct(bb)
+        return new
+
+    def window(self, hi: int) -> "GeometryState":
+        """The first ``hi`` molecules as views (no copy): work done on the
+        window, including the orbital update of the Jacobi warm start, lands
+        in this state's storage."""
+        import copy
+
+        new = copy.copy(self)
+        for name in self._PER_MOLECULE:
+            t = getattr(self, name)
+            if t is not None:
+                setattr(new, name, t[:hi])
+        ob = self.bb
+        dev = {k: (v if k in self._GLOBAL_DEV else v[:hi]) for k, v in ob.dev.items()}
+        host = {k: getattr(ob, k)[:hi] for k in self._BB_HOST}
+        new.bb = BatchBasis(ob.gfn, host["numbers"], host["nat"], host["nsh"], host["nao"], ob.nat_max, ob.nsh_max,
+                            ob.nao_max, host["sh_atom"], host["sh_ush"], host["sh_ao"], host["sh_l"], host["ao_atom"],
+                            host["ao_shell"], ob.item_mol[:0], ob.item_pair[:0], np.zeros(10, dtype=np.int32), dev)
         new.bs = _lib.basis_struct(new.bb)
         return new
+
+    def scatter(self, rows: torch.Tensor, other: "GeometryState") -> None:
+        """Write the molecules of ``other`` (same padded sizes) into the rows
+        ``rows`` of this state, in place (admission into serving slots).  The
+        warm-start orbitals of those rows restart from X."""
+        ob, nb = self.bb, other.bb
+        if (ob.nat_max, ob.nsh_max, ob.nao_max) != (nb.nat_max, nb.nsh_max, nb.nao_max) or ob.gfn != nb.gfn:
+            raise ValueError("scatter needs states of the same parametrization and padded sizes")
+        rows = rows.to(self.dev)
+        for name in self._PER_MOLECULE:
+            if name == "C":
+                continue
+            t = getattr(self, name)
+            if t is not None:
+                t.index_copy_(0, rows, getattr(other, name))
+        if self.C is None:
+            self.C = self.X.clone()
+        self.C.index_copy_(0, rows, other.X)
+        for k, v in ob.dev.items():
+            if k not in self._GLOBAL_DEV:
+                v.index_copy_(0, rows, nb.dev[k])
+        rc = rows.cpu().numpy()
+        for k in self._BB_HOST:
+            getattr(ob, k)[rc] = getattr(nb, k)
 
     # ---- one application of the SCF map -----------------------------------
     def unpack(self, x):
@@ -177,15 +230,18 @@ class GeometryState:
             A = (Cb.transpose(-1, -2) @ F @ Cb).contiguous()
             V = torch.empty_like(A)
             e = torch.empty((self.bb.nbatch, n), dtype=torch.float64, device=self.dev)
-            if self._jwork is None:
-                nbytes = self.lib.dxtb_jacobi_work_size(self.bb.nbatch, n)
+            nbytes = self.lib.dxtb_jacobi_work_size(self.bb.nbatch, n)
+            if self._jwork is None or self._jwork.numel() * 8 < nbytes:
                 self._jwork = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=self.dev)
             rc = self.lib.dxtb_jacobi_eigh(self.bb.nbatch, n, _lib.ptr(self.bb.dev["nao"]), _lib.ptr(A),
                                            _lib.ptr(V), _lib.ptr(e), _lib.ptr(self._jwork), 0, 50, JACOBI_TOL, None,
                                            _lib.stream_ptr(self.dev))
             _lib.check(rc, "dxtb_jacobi_eigh")
             C = torch.bmm(Cb, V.transpose(1, 2).contiguous())  # V holds the eigenvectors as rows
-            self.C = C
+            if self.C is None:
+                self.C = C
+            else:
+                self.C.copy_(C)  # in place: a window writes through to its parent state
         else:
             Fp = self.X.transpose(-1, -2) @ F @ self.X
             e, Cp = torch.linalg.eigh(Fp)
@@ -354,8 +410,12 @@ def run_scf(state: GeometryState, cull: float = 0.75) -> SCFResult:
             mix.C, mix.D = mix.C[keep], mix.D[keep]
     if bool(active.any()):
         x_full[idx[active]] = x[active]
-    x = x_full
-    # final state at the converged potential
+    return finalize(state, x_full, iters, conv)
+
+
+def finalize(state: GeometryState, x: torch.Tensor, iters: np.ndarray, conv: np.ndarray) -> SCFResult:
+    """Energies and the density of every molecule of ``state`` at its final
+    potential vector ``x`` (one more application of the SCF map)."""
     F = state.fock(x)
     P, e, C, occ, ts = state.solve(F)
     qsh, qat, qdp, qqp, eh0 = state.charges(P)

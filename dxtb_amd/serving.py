@@ -1,0 +1,356 @@
+"""
+Continuous-batching GFN1/GFN2 energy + forces engine (the serving loop).
+
+Requests (atomic numbers, positions, charge) are admitted into *slot pools*,
+one pool per basis-size class with fixed padded sizes.  An engine tick
+advances every pool that holds work by one SCF iteration over its occupied
+slots (one batched launch sequence per pool), retires the molecules that
+converged (energy and forces of all of them in one batched call) and admits
+waiting requests into the freed slots.  Requests that arrive while others
+are on the GPU join at the next tick; nothing waits for a batch to drain.
+
+A molecule's iteration sequence is exactly the one ``scf.run_scf`` runs for
+it alone: its potential vector, Broyden history (xitorch broyden1 as in the
+reference's scf/implicit path, exlibs/xitorch/_impls/optimize/root/
+rootsolver.py:42-300), iteration count and warm-start orbitals live in its
+slot, and every batched kernel treats each molecule independently.  So a
+result does not depend on which molecules shared the pool.
+
+Device memory of a pool is allocated once, when the pool is created (the
+arrays of a slot are overwritten on admission); each pool iterates only the
+window of rows up to its highest occupied slot.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import scf as hscf
+from .basis import build_batch, shell_table
+
+# basis-size classes (padded nao) of the slot pools
+POOL_NAO = (48, 96, 144, 192, 256, 320, 384, 448, 512, 640, 768, 1024, 1280, 1536)
+# padded atom and shell counts of a class, as fractions of its nao; a molecule
+# goes to the smallest class that holds all three of its sizes
+NAT_FRAC, NSH_FRAC = 0.5, 0.75
+
+
+@dataclass
+class Request:
+    rid: int
+    numbers: np.ndarray  # (nat,) atomic numbers
+    positions: np.ndarray  # (nat, 3) Bohr
+    charge: float = 0.0
+    arrival: float = 0.0  # seconds on the engine clock
+    admitted: float | None = None
+    finished: float | None = None
+    energy: float | None = None
+    forces: np.ndarray | None = None  # (nat, 3) -dE/dR, Eh/Bohr
+    iterations: int = 0
+    converged: bool = False
+
+    @property
+    def latency(self) -> float:
+        return float(self.finished - self.arrival)
+
+
+def sizes(numbers: np.ndarray, gfn: str) -> tuple[int, int, int]:
+    """(nat, nsh, nao) of a molecule."""
+    tab = shell_table(gfn)
+    z = np.asarray(numbers, dtype=np.int64)
+    z = z[z != 0]
+    if (z < 0).any() or (z >= len(tab.first)).any() or (tab.first[np.clip(z, 0, len(tab.first) - 1)] < 0).any():
+        raise ValueError(f"atomic numbers {sorted(set(z.tolist()))} include elements not parametrized in {gfn}")
+    if (tab.first[z] < 0).any():
+        raise ValueError(f"elements {sorted(set(z[tab.first[z] < 0].tolist()))} are not parametrized in {gfn}")
+    nsh = int(tab.count[z].sum())
+    nao = 0
+    for zz in z:
+        f = tab.first[zz]
+        nao += int((2 * tab.l[f : f + tab.count[zz]] + 1).sum())
+    return len(z), nsh, nao
+
+
+def class_dims(nao_max: int) -> tuple[int, int, int]:
+    """Padded (nat_max, nsh_max, nao_max) of the pool class ``nao_max``."""
+    return max(2, int(np.ceil(NAT_FRAC * nao_max))), max(2, int(np.ceil(NSH_FRAC * nao_max))), nao_max
+
+
+def pool_class(nat: int, nsh: int, nao: int) -> int:
+    """Smallest class holding a molecule of these sizes."""
+    for c in POOL_NAO:
+        na, ns, n = class_dims(c)
+        if nat <= na and nsh <= ns and nao <= n:
+            return c
+    raise ValueError(f"molecule too large for the serving pools (nao {nao} > {POOL_NAO[-1]})")
+
+
+def default_capacity(nao_max: int, budget_bytes: float = 8e9) -> int:
+    """Slots of a pool: as many as fill the device (256 workgroups of the
+    per-molecule kernels) within a memory budget per pool."""
+    na, ns, n = class_dims(nao_max)
+    per_slot = 8 * (24 * n * n + 2 * 251 * (ns + 9 * na))  # matrices + Broyden history (upper bound)
+    return int(max(8, min(256, budget_bytes // per_slot)))
+
+
+class SlotPool:
+    """Fixed-capacity slots of one basis-size class."""
+
+    def __init__(self, gfn: str, nao_max: int, capacity: int, device, opts: hscf.SCFOptions):
+        self.gfn = gfn
+        self.dims = class_dims(nao_max)
+        self.cap = capacity
+        self.dev = torch.device(device)
+        self.opts = opts
+        # every slot starts with a placeholder molecule (H2); empty slots are
+        # never iterated past the window and their results are never read
+        h2 = np.zeros((capacity, 2), dtype=np.int64) + 1
+        pos = np.zeros((capacity, self.dims[0], 3))
+        pos[:, 1, 2] = 1.4
+        bb = build_batch(h2, gfn, self.dev, self.dims)
+        self.state = hscf.GeometryState(bb, torch.as_tensor(pos, device=self.dev), opts=opts)
+        self.state.C = self.state.X.clone()
+        nbytes = self.state.lib.dxtb_jacobi_work_size(capacity, nao_max)
+        self.state._jwork = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=self.dev)  # shared by windows
+        L = self.state.nvec
+        H = opts.maxiter + 1
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        self.x = torch.zeros((capacity, L), **f64)
+        self.y = torch.zeros((capacity, L), **f64)
+        self.xprev = torch.zeros((capacity, L), **f64)
+        self.yprev = torch.zeros((capacity, L), **f64)
+        self.hc = torch.zeros((capacity, H, L), **f64)  # Broyden update vectors c_k
+        self.hd = torch.zeros((capacity, H, L), **f64)  # and d_k
+        self.k = np.zeros(capacity, dtype=np.int64)  # history length per slot
+        self.iters = np.zeros(capacity, dtype=np.int64)
+        self.occupied = np.zeros(capacity, dtype=bool)
+        self.fresh = np.zeros(capacity, dtype=bool)  # admitted, first map evaluation pending
+        self.req: list[Request | None] = [None] * capacity
+
+    @property
+    def nfree(self) -> int:
+        return int((~self.occupied).sum())
+
+    @property
+    def hi(self) -> int:
+        occ = np.nonzero(self.occupied)[0]
+        return int(occ[-1]) + 1 if len(occ) else 0
+
+    # ---- admission ---------------------------------------------------------
+    def admit(self, reqs: list[Request], now: float) -> None:
+        m = len(reqs)
+        if m == 0:
+            return
+        rows = np.nonzero(~self.occupied)[0][:m]
+        if len(rows) < m:
+            raise RuntimeError("not enough free slots")
+        na = self.dims[0]
+        N = np.zeros((m, na), dtype=np.int64)
+        P = np.zeros((m, na, 3))
+        for i, r in enumerate(reqs):
+            N[i, : len(r.numbers)] = r.numbers
+            P[i, : len(r.numbers)] = r.positions
+        bb = build_batch(N, self.gfn, self.dev, self.dims)
+        chg = np.array([r.charge for r in reqs], dtype=np.float64)
+        st = hscf.GeometryState(bb, torch.as_tensor(P, device=self.dev), charges=chg, opts=self.opts)
+        rt = torch.as_tensor(rows, device=self.dev)
+        self.state.scatter(rt, st)
+        for t in (self.x, self.y, self.xprev, self.yprev, self.hc, self.hd):
+            t.index_fill_(0, rt, 0.0)
+        self.k[rows] = 0
+        self.iters[rows] = 0
+        self.occupied[rows] = True
+        self.fresh[rows] = True
+        for r, row in zip(reqs, rows):
+            r.admitted = now
+            self.req[row] = r
+
+    # ---- one SCF iteration of every occupied slot --------------------------
+    def _mv(self, A, B, v, kmax):
+        r = -self.opts.alpha * v
+        if kmax == 0:
+            return r
+        coef = torch.bmm(B[:, :kmax], v.unsqueeze(-1))
+        return r + torch.bmm(A[:, :kmax].transpose(1, 2), coef).squeeze(-1)
+
+    def launch(self):
+        """Enqueue one iteration over the occupied window; returns what
+        ``finish`` needs (device tensors, no host synchronisation)."""
+        hi = self.hi
+        if hi == 0:
+            return None
+        o = self.opts
+        win = self.state.window(hi)
+        occ = torch.as_tensor(self.occupied[:hi], device=self.dev)
+        started = torch.as_tensor(self.occupied[:hi] & ~self.fresh[:hi], device=self.dev)
+        kmax = int(self.k[:hi][self.occupied[:hi]].max(initial=0))
+        x, y = self.x[:hi], self.y[:hi]
+        xprev, yprev = self.xprev[:hi], self.yprev[:hi]
+        hc, hd = self.hc[:hi], self.hd[:hi]
+        dx = -self._mv(hc, hd, y, kmax)
+        dxn = dx.norm(dim=-1)
+        xnew = torch.where(started.unsqueeze(-1), x + o.damp * dx, x)
+        ynew = torch.where(occ.unsqueeze(-1), xnew - win.g(xnew), y)
+        yn = ynew.norm(dim=-1)
+        # Broyden update of the started slots at their own history index
+        ddx, ddy = xnew - xprev, ynew - yprev
+        v = self._mv(hd, hc, ddx, kmax)
+        den = (ddy * v).sum(-1, keepdim=True)
+        ok = started.unsqueeze(-1) & (den != 0)
+        cnew = torch.where(ok, ddx - self._mv(hc, hd, ddy, kmax), 0.0)
+        dnew = torch.where(ok, v / torch.where(den == 0, 1.0, den), 0.0)
+        srows = np.nonzero(self.occupied[:hi] & ~self.fresh[:hi])[0]
+        if len(srows):
+            ri = torch.as_tensor(srows, device=self.dev)
+            ki = torch.as_tensor(self.k[srows], device=self.dev)
+            hc[ri, ki] = cnew[ri]
+            hd[ri, ki] = dnew[ri]
+        # the new point becomes the current one for every occupied slot
+        upd = occ.unsqueeze(-1)
+        x.copy_(torch.where(upd, xnew, x))
+        y.copy_(torch.where(upd, ynew, y))
+        xprev.copy_(torch.where(upd, xnew, xprev))
+        yprev.copy_(torch.where(upd, ynew, yprev))
+        done = started & (dxn < o.x_tol) & (yn < o.f_tol)
+        return hi, done, srows
+
+    def finish(self, pending, now_fn) -> list[Request]:
+        """Host bookkeeping of a launched iteration; retires finished slots."""
+        if pending is None:
+            return []
+        hi, done, srows = pending
+        done = done.cpu().numpy()
+        self.iters[srows] += 1
+        self.k[srows] += 1
+        full = srows[self.k[srows] >= self.hc.shape[1]]
+        if len(full):  # history full: restart it (never reached with H = maxiter + 1)
+            ft = torch.as_tensor(full, device=self.dev)
+            self.hc.index_fill_(0, ft, 0.0)
+            self.hd.index_fill_(0, ft, 0.0)
+            self.k[full] = 0
+        self.fresh[:hi] = False
+        out = np.nonzero(done | (self.occupied[:hi] & (self.iters[:hi] >= self.opts.maxiter)))[0]
+        if len(out) == 0:
+            return []
+        return self.retire(out, done[out], now_fn)
+
+    def retire(self, rows: np.ndarray, conv: np.ndarray, now_fn) -> list[Request]:
+        reqs = [self.req[r] for r in rows]
+        na = self.dims[0]
+        N = np.zeros((len(rows), na), dtype=np.int64)
+        for i, r in enumerate(reqs):
+            N[i, : len(r.numbers)] = r.numbers
+        bb = build_batch(N, self.gfn, self.dev, self.dims)
+        keep = torch.as_tensor(rows, device=self.dev)
+        sub = self.state.subset(keep, bb=bb)
+        res = hscf.finalize(sub, self.x.index_select(0, keep), self.iters[rows].copy(), conv.copy())
+        f = hscf.forces(res)
+        energy = res.energy.cpu().numpy()
+        frc = f.cpu().numpy()
+        t = now_fn()
+        for i, r in enumerate(reqs):
+            r.energy = float(energy[i])
+            r.forces = frc[i, : len(r.numbers)].copy()
+            r.iterations = int(self.iters[rows[i]])
+            r.converged = bool(conv[i])
+            r.finished = t
+        self.occupied[rows] = False
+        self.fresh[rows] = False
+        for r in rows:
+            self.req[r] = None
+        return reqs
+
+
+@dataclass
+class Engine:
+    """Continuous-batching engine over the slot pools of one parametrization."""
+
+    gfn: str = "gfn2"
+    device: str | torch.device = "cuda:0"
+    opts: hscf.SCFOptions = field(default_factory=hscf.SCFOptions)
+    capacity: dict | None = None  # class nao -> slots (default: default_capacity)
+    pools: dict = field(default_factory=dict)
+    queue: dict = field(default_factory=dict)  # class -> FIFO of requests
+    t0: float = field(default_factory=time.perf_counter)
+
+    def now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def pool(self, c: int) -> SlotPool:
+        if c not in self.pools:
+            cap = (self.capacity or {}).get(c, default_capacity(c))
+            self.pools[c] = SlotPool(self.gfn, c, cap, self.device, self.opts)
+        return self.pools[c]
+
+    def submit(self, r: Request) -> None:
+        c = pool_class(*sizes(r.numbers, self.gfn))
+        self.queue.setdefault(c, []).append(r)
+
+    @property
+    def busy(self) -> bool:
+        return any(p.occupied.any() for p in self.pools.values()) or any(self.queue.values())
+
+    def admit(self) -> None:
+        now = self.now()
+        for c, q in self.queue.items():
+            if not q:
+                continue
+            p = self.pool(c)
+            m = min(p.nfree, len(q))
+            if m:
+                p.admit(q[:m], now)
+                del q[:m]
+
+    def tick(self) -> list[Request]:
+        """Admit what fits, run one iteration of every pool, retire."""
+        self.admit()
+        launched = [(p, p.launch()) for p in self.pools.values()]
+        done = []
+        for p, pend in launched:
+            done += p.finish(pend, self.now)
+        return done
+
+    def run(self, requests: list[Request], realtime: bool = True) -> list[Request]:
+        """Serve ``requests`` (sorted by arrival).  With ``realtime`` a request
+        is submitted once the engine clock passes its arrival time; otherwise
+        all are submitted at once (offline batch)."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        self.t0 = time.perf_counter()
+        i = 0
+        out = []
+        while i < len(reqs) or self.busy:
+            now = self.now()
+            while i < len(reqs) and (not realtime or reqs[i].arrival <= now):
+                if not realtime:
+                    reqs[i].arrival = 0.0
+                self.submit(reqs[i])
+                i += 1
+            if not self.busy:
+                if i < len(reqs):
+                    time.sleep(max(0.0, min(reqs[i].arrival - self.now(), 0.01)))
+                continue
+            out += self.tick()
+        return out
+
+
+def plain_path(req: Request, gfn: str, device, opts: hscf.SCFOptions | None = None) -> Request:
+    """One molecule per call: setup, SCF and forces of a single request (the
+    baseline the serving form is measured against)."""
+    from .basis import build_batch as bb_
+
+    dev = torch.device(device)
+    N = np.asarray(req.numbers, dtype=np.int64)[None]
+    bb = bb_(N, gfn, dev)
+    st = hscf.GeometryState(bb, torch.as_tensor(np.asarray(req.positions)[None], device=dev),
+                            charges=np.array([req.charge]), opts=opts)
+    res = hscf.run_scf(st)
+    f = hscf.forces(res)
+    req.energy = float(res.energy[0])
+    req.forces = f[0, : len(req.numbers)].cpu().numpy()
+    req.iterations = int(res.iterations[0])
+    req.converged = bool(res.converged[0])
+    return req
