@@ -4,6 +4,12 @@ Benchmark of the MI355X GFN1/GFN2-xTB single-point path.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
 Workloads:
+  gfn2-batch -- (default) GFN2 energy + forces of 1024 seeded molecules of
+                10-120 atoms through the continuous-batching engine;
+                molecules/s with p50/p99 completion latency.
+  serving    -- serving stream: log-normal molecule sizes (median 40 atoms),
+                Poisson arrivals at three offered loads; molecules/s and
+                p50/p99 latency, against the plain one-molecule-per-call path.
   integrals  -- integral-driver microbenchmark: overlap + dipole + quadrupole
                 and their d/dR VJP for a padded batch of 512 GFN2 molecules of
                 60 atoms (H:C:N:O:S = .45:.3:.1:.1:.05); rate in shell pairs/s.
@@ -151,82 +157,225 @@ def cpu_baseline_integrals(nums, poss, budget_s=15.0):
     }
 
 
+def jacobi_flops_per_sweep(n: int, variant: int) -> float:
+    """Algorithmic FP64 operations of one sweep of the Jacobi solver on an
+    order-n matrix (every rotation applied).  Scalar variant: n(n-1)/2
+    rotations, each updating two rows and two columns of A (24 operations per
+    2x2 block touched) and two rows of V (6 per element pair).  Blocked
+    variant: per outer step the 32x32 subproblem sweeps, the products
+    Q_k^T A_kk' Q_k' of the off-diagonal tiles and Q_k^T VT of the panels."""
+    if n < 2:
+        return 0.0
+    if variant == 1:
+        m = n + (n & 1)
+        return (m - 1) * ((m // 2) ** 2 * 24.0 + (m // 2) * n * 6.0)
+    npairs = -(-n // 32)
+    nb = 2 * npairs
+    ntile = npairs * (npairs - 1) // 2
+    npanel = -(-n // 32)
+    gemm = 2.0 * 32 ** 3
+    inner = (256 * 24.0 + 512 * 6.0)  # one subproblem step
+    steps_inner = 31 + 16 * (nb - 2)
+    return (nb - 1) * (ntile * 2 * gemm + npairs * npanel * gemm) + npairs * steps_inner * inner
+
+
+def jacobi_roofline(records) -> dict:
+    """Achieved FP64 rate of the Jacobi launches recorded in the timed region
+    (HIP events on the launch stream; flops from the sweeps each matrix ran)."""
+    if not records:
+        return None
+    t_ms = []
+    flops = 0.0
+    nsweep, nmat = 0, 0
+    by_var = {1: [0.0, 0.0], 2: [0.0, 0.0]}
+    for e0, e1, n, nao, sw, variant in records:
+        ms = e0.elapsed_time(e1)
+        t_ms.append(ms)
+        sweeps = sw.cpu().numpy()
+        nsweep += int(sweeps.sum())
+        nmat += len(sweeps)
+        f = float(sum(int(sweeps[i]) * jacobi_flops_per_sweep(int(nao[i]), variant) for i in range(len(sweeps))))
+        flops += f
+        by_var[variant][0] += f
+        by_var[variant][1] += ms
+    total_ms = float(np.sum(t_ms))
+    achieved = flops / (total_ms * 1e-3) / 1e12
+    return {
+        "kernel": "k_jacobi_blocked / k_jacobi_scalar (dxtb_jacobi_eigh)",
+        "bound": "mfma",
+        "achieved": achieved,
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / FP64_PEAK_TFLOPS,
+        "traffic": None,
+        "launches": len(t_ms),
+        "avg_launch_ms": total_ms / len(t_ms),
+        "total_kernel_ms": total_ms,
+        "mean_sweeps": nsweep / max(nmat, 1),
+        "blocked_tflops": by_var[2][0] / max(by_var[2][1], 1e-9) / 1e9,
+        "scalar_tflops": by_var[1][0] / max(by_var[1][1], 1e-9) / 1e9,
+    }
+
+
+def _progress(label):
+    return lambda done, total: print(f"{label}: {done}/{total} done", file=sys.stderr, flush=True)
+
+
+def _requests(nums, poss, chg, idx):
+    from dxtb_amd import serving as sv
+
+    return [sv.Request(int(i), nums[i], poss[i], float(chg[i])) for i in idx]
+
+
+def _stratified(nums, gfn, per_class):
+    """Indices of up to per_class molecules of every pool class (warm-up:
+    creates every pool the workload needs)."""
+    from dxtb_amd import serving as sv
+
+    seen = {}
+    for i, z in enumerate(nums):
+        c = sv.pool_class(*sv.sizes(z, gfn))
+        seen.setdefault(c, [])
+        if len(seen[c]) < per_class:
+            seen[c].append(i)
+    return sorted(i for v in seen.values() for i in v)
+
+
 def bench_gfn2_batch(args, dev):
-    """Config 2: GFN2 energy + forces for a seeded set of molecules of 10-120
-    atoms, bucketed by basis size, all molecules of a bucket in one padded
-    launch sequence (integrals, SCF, forces)."""
-    from dxtb_amd import basis as hbasis
+    """Config 2: GFN2 energy + forces of 1024 seeded molecules of 10-120 atoms
+    (H/C/N/O/F/S), all submitted at once to the continuous-batching engine
+    (slot pools by basis size; converged molecules leave, waiting ones enter
+    between SCF iterations)."""
     from dxtb_amd import scf as hscf
+    from dxtb_amd import serving as sv
     from dxtb_amd import workloads
 
-    nums, poss, _ = workloads.config_batch(2, count=args.nmol)
-    tab = hbasis.shell_table("gfn2")
-    ao_per_z = np.zeros(len(tab.first), dtype=np.int64)
-    for z in range(1, len(tab.first)):
-        if tab.first[z] >= 0:
-            ao_per_z[z] = int((2 * tab.l[tab.first[z] : tab.first[z] + tab.count[z]] + 1).sum())
-    nao = np.array([int(ao_per_z[np.asarray(zz)].sum()) for zz in nums])
-    order = np.argsort(nao, kind="stable")
-    buckets = [order[i : i + args.bucket] for i in range(0, len(order), args.bucket)]
-    prepared = []
-    for idx in buckets:
-        N, P = workloads.pad([nums[i] for i in idx], [poss[i] for i in idx])
-        bb = hbasis.build_batch(N, "gfn2", dev)
-        prepared.append((idx, bb, torch.as_tensor(P, dtype=torch.float64, device=dev)))
-
-    def run_all(record=None):
-        out = []
-        for idx, bb, pos in prepared:
-            t0 = time.perf_counter()
-            st = hscf.GeometryState(bb, pos)
-            res = hscf.run_scf(st)
-            f = hscf.forces(res)
-            torch.cuda.synchronize(dev)
-            dt = time.perf_counter() - t0
-            if record is not None:
-                record.append((len(idx), dt, int(res.iterations.max()), bool(res.converged.all())))
-                print(f"bucket n={len(idx)} nao_max={bb.nao_max} {dt*1e3:.1f} ms "
-                      f"iters max={int(res.iterations.max())} mean={res.iterations.mean():.1f}",
-                      file=sys.stderr, flush=True)
-            out.append((res.energy, f))
-        return out
-
+    nums, poss, chg = workloads.config_batch(2, count=args.nmol)
+    eng = sv.Engine("gfn2", dev)
+    warm = _stratified(nums, "gfn2", 8)
     for _ in range(args.warmup):
-        run_all()
+        eng.run(_requests(nums, poss, chg, warm), realtime=False, progress=_progress("warm-up"))
     torch.cuda.synchronize(dev)
-    recs = []
-    t0 = time.perf_counter()
+    walls, out = [], []
+    hscf.JACOBI_PROFILE = []
     for _ in range(args.steps):
-        recs.clear()
-        run_all(recs)
-    torch.cuda.synchronize(dev)
-    wall = (time.perf_counter() - t0) / args.steps
-    lat = np.concatenate([[dt] * n for n, dt, _, _ in recs])
+        reqs = _requests(nums, poss, chg, range(len(nums)))
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        out = eng.run(reqs, realtime=False, progress=_progress("config-2 batch"))
+        torch.cuda.synchronize(dev)
+        walls.append(time.perf_counter() - t0)
+    roof = jacobi_roofline(hscf.JACOBI_PROFILE)
+    hscf.JACOBI_PROFILE = None
+    wall = float(np.mean(walls))
+    lat = np.array([r.latency for r in out])
+    its = np.array([r.iterations for r in out])
     res = {
         "metric": "GFN2 energy+forces molecules/s, p50/p99 latency at |dE|<1e-8 Eh",
-        "value": args.nmol / wall,
+        "value": len(out) / wall,
         "unit": "molecules/s",
         "ms_per_step": wall * 1e3,
         "dtype": "f64",
         "config": {"workload": f"config-2 batched GFN2 energy+forces: {args.nmol} molecules, 10-120 atoms, "
-                               f"buckets of {args.bucket} by basis size",
+                               "H/C/N/O/F/S, all submitted at t=0 to the continuous-batching engine",
                    "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
                    "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
-                   "max_scf_iterations": int(max(r[2] for r in recs)),
-                   "all_converged": bool(all(r[3] for r in recs))},
+                   "scf_iterations_mean": float(its.mean()),
+                   "scf_iterations_max": int(its.max()),
+                   "converged": int(sum(r.converged for r in out)),
+                   "molecules": len(out)},
+        "roofline": roof,
     }
-    return res, (nums, poss)
+    return res, (nums, poss, chg, "config-2")
 
 
-def cpu_baseline_gfn2(nums, poss, budget_s=20.0):
+def bench_serving(args, dev):
+    """Config 3: serving stream of seeded requests (log-normal sizes, median
+    40 atoms, 5-400; H/C/N/O/F/S/Cl; charges 0/+1/-1 at 80/10/10) with
+    Poisson arrivals at three offered loads (30/60/90 % of the capacity the
+    engine shows on an offline batch of the same distribution); compared
+    with the plain one-molecule-per-call GPU path."""
+    from dxtb_amd import scf as hscf
+    from dxtb_amd import serving as sv
+    from dxtb_amd import workloads
+
+    nums, poss, chg = workloads.config_batch(3, count=args.nmol)
+    n = len(nums)
+    eng = sv.Engine("gfn2", dev)
+    for _ in range(args.warmup):
+        eng.run(_requests(nums, poss, chg, _stratified(nums, "gfn2", 4)), realtime=False,
+                progress=_progress("warm-up"))
+    # capacity: offline batch of the first third of the requests
+    cap_idx = range(0, n // 3)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    cap_out = eng.run(_requests(nums, poss, chg, cap_idx), realtime=False, progress=_progress("capacity batch"))
+    capacity = len(cap_out) / (time.perf_counter() - t0)
+    # streams at three offered loads, each on its own slice of the requests
+    rng = np.random.default_rng(workloads.seed(3) + 1)
+    per = (n - n // 3) // 3
+    loads = []
+    hscf.JACOBI_PROFILE = []
+    walls = []
+    for j, frac in enumerate((0.3, 0.6, 0.9)):
+        idx = range(n // 3 + j * per, n // 3 + (j + 1) * per)
+        reqs = _requests(nums, poss, chg, idx)
+        rate = frac * capacity
+        t = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
+        for r, a in zip(reqs, t):
+            r.arrival = float(a)
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        out = eng.run(reqs, realtime=True, progress=_progress(f"stream at {frac:.0%} load"))
+        wall = time.perf_counter() - t0
+        walls.append(wall)
+        lat = np.array([r.latency for r in out])
+        loads.append({"offered_mol_s": rate, "achieved_mol_s": len(out) / wall, "requests": len(out),
+                      "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
+                      "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
+                      "converged": int(sum(r.converged for r in out))})
+    roof = jacobi_roofline(hscf.JACOBI_PROFILE)
+    hscf.JACOBI_PROFILE = None
+    # plain path: one molecule per call, on a sample of the same stream
+    sample = _requests(nums, poss, chg, range(n // 3, n // 3 + args.plain_sample))
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for k, r in enumerate(sample):
+        sv.plain_path(r, "gfn2", dev)
+        if k % 8 == 7:
+            print(f"plain path: {k + 1}/{len(sample)}", file=sys.stderr, flush=True)
+    plain = len(sample) / (time.perf_counter() - t0)
+    top = loads[-1]
+    res = {
+        "metric": "GFN2 energy+forces molecules/s, p50/p99 latency at |dE|<1e-8 Eh",
+        "value": top["achieved_mol_s"],
+        "unit": "molecules/s",
+        "ms_per_step": walls[-1] * 1e3,
+        "dtype": "f64",
+        "config": {"workload": f"config-3 serving stream: {n} seeded requests (log-normal sizes, median 40 atoms, "
+                               "5-400, H/C/N/O/F/S/Cl, charges 0/+-1), Poisson arrivals at 30/60/90% of the "
+                               "offline capacity; value = throughput at the 90% load; no implicit solvation",
+                   "offline_capacity_mol_s": capacity,
+                   "loads": loads,
+                   "p50_latency_ms": top["p50_latency_ms"],
+                   "p99_latency_ms": top["p99_latency_ms"],
+                   "plain_path_mol_s": plain,
+                   "plain_sample": len(sample),
+                   "serving_over_plain": capacity / plain},
+        "roofline": roof,
+    }
+    return res, (nums, poss, chg, "config-3")
+
+
+def cpu_baseline_gfn2(nums, poss, chg, label, budget_s=20.0):
     from oracle import scf as oscf
 
     torch.set_num_threads(os.cpu_count() or 1)
     t0 = time.perf_counter()
     k = 0
     while k < len(nums) and time.perf_counter() - t0 < budget_s:
-        oscf.singlepoint(nums[k].tolist(), torch.tensor(poss[k], dtype=torch.float64), "gfn2", tol=1e-10,
-                         forces=True)
+        oscf.singlepoint(nums[k].tolist(), torch.tensor(poss[k], dtype=torch.float64), "gfn2", charge=float(chg[k]),
+                         tol=1e-10, forces=True)
         k += 1
     dt = time.perf_counter() - t0
     return {
@@ -234,7 +383,7 @@ def cpu_baseline_gfn2(nums, poss, budget_s=20.0):
         "unit": "molecules/s",
         "cores": torch.get_num_threads(),
         "kind": "port",
-        "sample": f"first {k} config-2 molecules (seeded), GFN2 energy+forces, FP64 torch CPU oracle "
+        "sample": f"first {k} {label} molecules (seeded), GFN2 energy+forces, FP64 torch CPU oracle "
                   "(xitorch-style broyden1, LAPACK eigh, variational autograd forces)",
     }
 
@@ -242,11 +391,11 @@ def cpu_baseline_gfn2(nums, poss, budget_s=20.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="gfn2-batch", choices=["gfn2-batch", "integrals"])
+    ap.add_argument("--workload", default="gfn2-batch", choices=["gfn2-batch", "serving", "integrals"])
     ap.add_argument("--nmol", type=int, default=None)
-    ap.add_argument("--bucket", type=int, default=128)
+    ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -259,6 +408,9 @@ def main():
     if args.workload == "integrals":
         args.nmol = args.nmol or 512
         res, extra = bench_integrals(args, dev)
+    elif args.workload == "serving":
+        args.nmol = args.nmol or 1200
+        res, extra = bench_serving(args, dev)
     else:
         args.nmol = args.nmol or 1024
         res, extra = bench_gfn2_batch(args, dev)
@@ -268,7 +420,7 @@ def main():
         if args.workload == "integrals":
             res["cpu_baseline"] = cpu_baseline_integrals(extra[0], extra[1])
         else:
-            res["cpu_baseline"] = cpu_baseline_gfn2(extra[0], extra[1])
+            res["cpu_baseline"] = cpu_baseline_gfn2(*extra)
     print(json.dumps(res), flush=True)
 
 

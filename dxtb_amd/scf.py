@@ -39,6 +39,13 @@ PAD_DIAG = 1.0e3  # Fock diagonal of padding AOs: above every real level
 # errors are then O(1e-12 Eh / gap), far below what the 1e-8 Eh energy and
 # 1e-6 Eh/Bohr force bars can see (the energy is variational in the orbitals).
 JACOBI_TOL = 1.0e-12
+# orders up to this use the scalar LDS variant of the Jacobi solver, larger
+# ones the blocked matrix-core variant
+JACOBI_SCALAR_MAX = 64
+# when a list, solve() appends (start event, end event, order, basis sizes,
+# sweeps tensor, variant) for every Jacobi launch (kernel timing on the
+# launch stream; used by the benchmark)
+JACOBI_PROFILE: list | None = None
 
 
 @dataclass
@@ -50,6 +57,19 @@ class SCFOptions:
     damp: float = 0.5
     alpha: float = -0.5
     eigensolver: str = "jacobi"  # "jacobi": hand-written Jacobi for nao <= 512, else device eigh
+
+
+def _orthogonaliser(spad: torch.Tensor, chunk: int = 32) -> torch.Tensor:
+    """X = L^-T with S = L L^T (so X^T S X = 1), in chunks of matrices: the
+    device batched triangular solve fails for large batch counts."""
+    B, n, _ = spad.shape
+    X = torch.empty_like(spad)
+    eye = torch.eye(n, dtype=spad.dtype, device=spad.device)
+    for i in range(0, B, chunk):
+        L = torch.linalg.cholesky(spad[i : i + chunk])
+        Y = torch.linalg.solve_triangular(L, eye.expand(L.shape[0], n, n), upper=False)
+        X[i : i + chunk] = Y.transpose(-1, -2)
+    return X
 
 
 class GeometryState:
@@ -85,9 +105,7 @@ class GeometryState:
         spad = self.S.clone()
         pad = torch.arange(n, device=dev)[None, :] >= d["nao"][:, None]
         spad.diagonal(dim1=-2, dim2=-1).copy_(torch.where(pad, 1.0, spad.diagonal(dim1=-2, dim2=-1)))
-        L = torch.linalg.cholesky(spad)
-        eye = torch.eye(n, **f64).expand(B, n, n)
-        self.X = torch.linalg.solve_triangular(L, eye, upper=False).transpose(-1, -2).contiguous()
+        self.X = _orthogonaliser(spad)
         # reference occupations and electron counts (scf/iterator.py:147-195)
         tab_occ = _refocc_table(bb.gfn)
         n0 = np.zeros((B, n))
@@ -233,10 +251,20 @@ class GeometryState:
             nbytes = self.lib.dxtb_jacobi_work_size(self.bb.nbatch, n)
             if self._jwork is None or self._jwork.numel() * 8 < nbytes:
                 self._jwork = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=self.dev)
+            variant = 1 if n <= JACOBI_SCALAR_MAX else 2
+            prof = JACOBI_PROFILE
+            sw = None
+            if prof is not None:
+                sw = torch.zeros(self.bb.nbatch, dtype=torch.int32, device=self.dev)
+                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                ev[0].record(torch.cuda.current_stream(self.dev))
             rc = self.lib.dxtb_jacobi_eigh(self.bb.nbatch, n, _lib.ptr(self.bb.dev["nao"]), _lib.ptr(A),
-                                           _lib.ptr(V), _lib.ptr(e), _lib.ptr(self._jwork), 0, 50, JACOBI_TOL, None,
-                                           _lib.stream_ptr(self.dev))
+                                           _lib.ptr(V), _lib.ptr(e), _lib.ptr(self._jwork), variant, 50, JACOBI_TOL,
+                                           _lib.ptr(sw), _lib.stream_ptr(self.dev))
             _lib.check(rc, "dxtb_jacobi_eigh")
+            if prof is not None:
+                ev[1].record(torch.cuda.current_stream(self.dev))
+                prof.append((ev[0], ev[1], n, np.array(self.bb.nao), sw, variant))
             C = torch.bmm(Cb, V.transpose(1, 2).contiguous())  # V holds the eigenvectors as rows
             if self.C is None:
                 self.C = C
@@ -244,7 +272,12 @@ class GeometryState:
                 self.C.copy_(C)  # in place: a window writes through to its parent state
         else:
             Fp = self.X.transpose(-1, -2) @ F @ self.X
+            side = torch.cuda.current_stream(self.dev) != torch.cuda.default_stream(self.dev)
+            if side:  # the device solver is not reliably ordered with a side stream
+                torch.cuda.synchronize(self.dev)
             e, Cp = torch.linalg.eigh(Fp)
+            if side:
+                torch.cuda.synchronize(self.dev)
             C = self.X @ Cp
         occ = torch.empty((self.bb.nbatch, 2, self.bb.nao_max), dtype=torch.float64, device=self.dev)
         ts = torch.empty(self.bb.nbatch, dtype=torch.float64, device=self.dev)

@@ -106,6 +106,9 @@ class SlotPool:
         self.cap = capacity
         self.dev = torch.device(device)
         self.opts = opts
+        # all work of the pool runs on its own stream (default priority), so
+        # the iterations of different pools overlap on the device
+        self.stream = torch.cuda.Stream(self.dev)
         # every slot starts with a placeholder molecule (H2); empty slots are
         # never iterated past the window and their results are never read
         h2 = np.zeros((capacity, 2), dtype=np.int64) + 1
@@ -130,6 +133,8 @@ class SlotPool:
         self.occupied = np.zeros(capacity, dtype=bool)
         self.fresh = np.zeros(capacity, dtype=bool)  # admitted, first map evaluation pending
         self.req: list[Request | None] = [None] * capacity
+        self.pending = None  # launched iteration awaiting finish()
+        self.event = torch.cuda.Event()
 
     @property
     def nfree(self) -> int:
@@ -185,8 +190,15 @@ class SlotPool:
             return None
         o = self.opts
         win = self.state.window(hi)
-        occ = torch.as_tensor(self.occupied[:hi], device=self.dev)
-        started = torch.as_tensor(self.occupied[:hi] & ~self.fresh[:hi], device=self.dev)
+        # every host-to-device copy comes before the iteration is enqueued (a
+        # copy from pageable memory waits for the stream)
+        srows = np.nonzero(self.occupied[:hi] & ~self.fresh[:hi])[0]
+        idx = torch.as_tensor(np.concatenate([self.occupied[:hi], self.occupied[:hi] & ~self.fresh[:hi]]).astype(
+            np.int64), device=self.dev)
+        occ, started = idx[:hi].bool(), idx[hi:].bool()
+        if len(srows):
+            rk = torch.as_tensor(np.stack([srows, self.k[srows]]), device=self.dev)
+            ri, ki = rk[0], rk[1]
         kmax = int(self.k[:hi][self.occupied[:hi]].max(initial=0))
         x, y = self.x[:hi], self.y[:hi]
         xprev, yprev = self.xprev[:hi], self.yprev[:hi]
@@ -203,10 +215,7 @@ class SlotPool:
         ok = started.unsqueeze(-1) & (den != 0)
         cnew = torch.where(ok, ddx - self._mv(hc, hd, ddy, kmax), 0.0)
         dnew = torch.where(ok, v / torch.where(den == 0, 1.0, den), 0.0)
-        srows = np.nonzero(self.occupied[:hi] & ~self.fresh[:hi])[0]
         if len(srows):
-            ri = torch.as_tensor(srows, device=self.dev)
-            ki = torch.as_tensor(self.k[srows], device=self.dev)
             hc[ri, ki] = cnew[ri]
             hd[ri, ki] = dnew[ri]
         # the new point becomes the current one for every occupied slot
@@ -283,7 +292,9 @@ class Engine:
     def pool(self, c: int) -> SlotPool:
         if c not in self.pools:
             cap = (self.capacity or {}).get(c, default_capacity(c))
-            self.pools[c] = SlotPool(self.gfn, c, cap, self.device, self.opts)
+            p = SlotPool(self.gfn, c, cap, self.device, self.opts)
+            torch.cuda.synchronize(p.dev)  # pool arrays were filled on the creating stream
+            self.pools[c] = p
         return self.pools[c]
 
     def submit(self, r: Request) -> None:
@@ -292,36 +303,66 @@ class Engine:
 
     @property
     def busy(self) -> bool:
-        return any(p.occupied.any() for p in self.pools.values()) or any(self.queue.values())
+        return (any(p.occupied.any() or p.pending is not None for p in self.pools.values())
+                or any(self.queue.values()))
 
     def admit(self) -> None:
+        """Admit waiting requests into idle pools (no iteration in flight).
+        The setup runs on the default stream (it uses the vendor solvers,
+        Cholesky and triangular solve, which are not reliably ordered with a
+        side stream); the pool's stream then waits for it."""
         now = self.now()
+        dflt = torch.cuda.default_stream(torch.device(self.device))
         for c, q in self.queue.items():
             if not q:
                 continue
             p = self.pool(c)
+            if p.pending is not None:
+                continue
             m = min(p.nfree, len(q))
             if m:
-                p.admit(q[:m], now)
+                with torch.cuda.stream(dflt):
+                    p.admit(q[:m], now)
+                p.stream.wait_stream(dflt)
                 del q[:m]
 
-    def tick(self) -> list[Request]:
-        """Admit what fits, run one iteration of every pool, retire."""
+    def step(self) -> list[Request]:
+        """One pass of the event loop: admit what fits, finish every pool
+        whose iteration has completed (retiring converged molecules) and
+        launch its next iteration.  Pools advance independently: a pool of
+        small molecules does not wait for a pool of large ones."""
         self.admit()
-        launched = [(p, p.launch()) for p in self.pools.values()]
         done = []
-        for p, pend in launched:
-            done += p.finish(pend, self.now)
+        for p in self.pools.values():
+            if p.pending is not None and p.event.query():
+                with torch.cuda.stream(p.stream):
+                    done += p.finish(p.pending, self.now)
+                p.pending = None
+            if p.pending is None and p.occupied.any():
+                with torch.cuda.stream(p.stream):
+                    p.pending = p.launch()
+                    p.event.record(p.stream)
         return done
 
-    def run(self, requests: list[Request], realtime: bool = True) -> list[Request]:
+    def tick(self) -> list[Request]:
+        """One synchronous iteration of every pool (admit, launch all, finish
+        all)."""
+        out = self.step()
+        for p in self.pools.values():
+            if p.pending is not None:
+                p.event.synchronize()
+        return out + self.step()
+
+    def run(self, requests: list[Request], realtime: bool = True, progress=None) -> list[Request]:
         """Serve ``requests`` (sorted by arrival).  With ``realtime`` a request
         is submitted once the engine clock passes its arrival time; otherwise
-        all are submitted at once (offline batch)."""
+        all are submitted at once (offline batch).  ``progress(done, total)``
+        is called whenever another 10 % of the requests has completed."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         self.t0 = time.perf_counter()
         i = 0
         out = []
+        mark = max(1, len(reqs) // 10)
         while i < len(reqs) or self.busy:
             now = self.now()
             while i < len(reqs) and (not realtime or reqs[i].arrival <= now):
@@ -333,7 +374,10 @@ class Engine:
                 if i < len(reqs):
                     time.sleep(max(0.0, min(reqs[i].arrival - self.now(), 0.01)))
                 continue
-            out += self.tick()
+            n0 = len(out)
+            out += self.step()
+            if progress is not None and len(out) // mark > n0 // mark:
+                progress(len(out), len(reqs))
         return out
 
 
