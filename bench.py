@@ -221,10 +221,15 @@ def _progress(label):
     return lambda done, total: print(f"{label}: {done}/{total} done", file=sys.stderr, flush=True)
 
 
-def _requests(nums, poss, chg, idx):
+def _requests(nums, poss, chg, idx, solv=None):
+    """Requests for molecules ``idx``; where ``solv`` is set, with GBSA
+    (generalized Born, P16 kernel, eps 78.4, no ALPB term)."""
     from dxtb_amd import serving as sv
+    from dxtb_amd.solvation import GeneralizedBorn
 
-    return [sv.Request(int(i), nums[i], poss[i], float(chg[i])) for i in idx]
+    gb = GeneralizedBorn(None, 78.4, alpb=False)
+    return [sv.Request(int(i), nums[i], poss[i], float(chg[i]),
+                       solvation=gb if solv is not None and solv[i] else None) for i in idx]
 
 
 def _stratified(nums, gfn, per_class):
@@ -300,16 +305,18 @@ def bench_serving(args, dev):
     from dxtb_amd import workloads
 
     nums, poss, chg = workloads.config_batch(3, count=args.nmol)
+    solv = workloads.config3_solvation(len(nums))
     n = len(nums)
     eng = sv.Engine("gfn2", dev)
     for _ in range(args.warmup):
-        eng.run(_requests(nums, poss, chg, _stratified(nums, "gfn2", 4)), realtime=False,
+        eng.run(_requests(nums, poss, chg, _stratified(nums, "gfn2", 4), solv), realtime=False,
                 progress=_progress("warm-up"))
     # capacity: offline batch of the first third of the requests
     cap_idx = range(0, n // 3)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
-    cap_out = eng.run(_requests(nums, poss, chg, cap_idx), realtime=False, progress=_progress("capacity batch"))
+    cap_out = eng.run(_requests(nums, poss, chg, cap_idx, solv), realtime=False,
+                      progress=_progress("capacity batch"))
     capacity = len(cap_out) / (time.perf_counter() - t0)
     # streams at three offered loads, each on its own slice of the requests
     rng = np.random.default_rng(workloads.seed(3) + 1)
@@ -319,7 +326,7 @@ def bench_serving(args, dev):
     walls = []
     for j, frac in enumerate((0.3, 0.6, 0.9)):
         idx = range(n // 3 + j * per, n // 3 + (j + 1) * per)
-        reqs = _requests(nums, poss, chg, idx)
+        reqs = _requests(nums, poss, chg, idx, solv)
         rate = frac * capacity
         t = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
         for r, a in zip(reqs, t):
@@ -337,7 +344,7 @@ def bench_serving(args, dev):
     roof = jacobi_roofline(hscf.JACOBI_PROFILE)
     hscf.JACOBI_PROFILE = None
     # plain path: one molecule per call, on a sample of the same stream
-    sample = _requests(nums, poss, chg, range(n // 3, n // 3 + args.plain_sample))
+    sample = _requests(nums, poss, chg, range(n // 3, n // 3 + args.plain_sample), solv)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     for k, r in enumerate(sample):
@@ -354,11 +361,12 @@ def bench_serving(args, dev):
         "dtype": "f64",
         "config": {"workload": f"config-3 serving stream: {n} seeded requests (log-normal sizes, median 40 atoms, "
                                "5-400, H/C/N/O/F/S/Cl, charges 0/+-1), Poisson arrivals at 30/60/90% of the "
-                               "offline capacity; value = throughput at the 90% load; no implicit solvation",
+                               "offline capacity; 20% of requests with GBSA (eps 78.4); value = throughput at the 90% load",
                    "offline_capacity_mol_s": capacity,
                    "loads": loads,
                    "p50_latency_ms": top["p50_latency_ms"],
                    "p99_latency_ms": top["p99_latency_ms"],
+                   "gbsa_requests": int(solv.sum()),
                    "plain_path_mol_s": plain,
                    "plain_sample": len(sample),
                    "serving_over_plain": capacity / plain},

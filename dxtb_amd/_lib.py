@@ -72,6 +72,13 @@ class DxtbParams(ctypes.Structure):
 
 
 # name -> argtypes; every entry point returns int
+class DxtbGbParams(ctypes.Structure):
+    """dxtb_gb_params_t"""
+
+    _fields_ = [("descreening", c_double), ("cutoff", c_double), ("born_scale", c_double), ("born_offset", c_double),
+                ("obc", c_double * 3)]
+
+
 _SIGNATURES: dict[str, list] = {
     "dxtb_multipole_fwd": [POINTER(DxtbBasis)] + [c_void_p] * 6 + [c_int, c_void_p],
     "dxtb_multipole_vjp": [POINTER(DxtbBasis), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
@@ -82,6 +89,9 @@ _SIGNATURES: dict[str, list] = {
     "dxtb_fermi": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p],
     "dxtb_forces_multipole": [POINTER(DxtbBasis)] + [c_void_p] * 9,
     "dxtb_jacobi_max_n": [],
+    "dxtb_gb_born": [c_int, c_int] + [c_void_p] * 5 + [POINTER(DxtbGbParams)] + [c_void_p] * 3,
+    "dxtb_gb_potential": [c_int, c_int] + [c_void_p] * 9,
+    "dxtb_gb_forces": [c_int, c_int] + [c_void_p] * 6 + [POINTER(DxtbGbParams)] + [c_void_p] * 5,
     "dxtb_jacobi_work_size": [c_int, c_int],
     "dxtb_jacobi_eigh": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double,
                          c_void_p, c_void_p],

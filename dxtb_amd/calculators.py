@@ -28,6 +28,7 @@ import torch
 
 from . import scf as _scf
 from .basis import build_batch
+from .solvation import GeneralizedBorn
 
 
 class DtypeError(ValueError):
@@ -72,8 +73,8 @@ def _opts_to_scf(opts: dict | None) -> _scf.SCFOptions:
 class _Calculator:
     gfn = ""
 
-    def __init__(self, numbers: torch.Tensor, *, opts: dict | None = None, device=None,
-                 dtype: torch.dtype = torch.float64, **_):
+    def __init__(self, numbers: torch.Tensor, *, classical=None, interaction=None, opts: dict | None = None,
+                 device=None, dtype: torch.dtype = torch.float64, **_):
         if numbers.dtype not in (torch.int16, torch.int32, torch.int64, torch.long):
             raise DtypeError(f"Atomic numbers must be an integer tensor, got {numbers.dtype}.")
         if dtype != torch.float64:
@@ -89,6 +90,14 @@ class _Calculator:
         self.scf_opts = _opts_to_scf(opts)
         self.force_convergence = bool(self.opts.get("force_convergence", False))
         self.bb = build_batch(nums, self.gfn, self.device)
+        # extra interactions (calculators/types/base.py: `interaction` list):
+        # generalized Born implicit solvation is supported
+        self.solvation = None
+        for it in (interaction or []):
+            if isinstance(it, GeneralizedBorn):
+                self.solvation = it
+            else:
+                raise NotImplementedError(f"interaction {type(it).__name__} is not supported on this path")
         self.cache: dict = {}
         self._ncalcs = 0
 
@@ -123,7 +132,7 @@ class _Calculator:
         if spin not in (None, 0):
             raise NotImplementedError("explicit spin is not supported on this path")
         pos = positions.detach().reshape(self.bb.nbatch, self.bb.nat_max, 3)
-        state = _scf.GeometryState(self.bb, pos, self._charges(chrg), self.scf_opts)
+        state = _scf.GeometryState(self.bb, pos, self._charges(chrg), self.scf_opts, solvation=self.solvation)
         res = _scf.run_scf(state)
         if not res.converged.all():
             msg = f"SCF did not converge for {int((~res.converged).sum())} molecule(s)."

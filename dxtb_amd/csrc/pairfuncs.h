@@ -154,4 +154,74 @@ DXTB_HD double dmrad_dcn(double cn, double rad, double vcn, double kexp, double 
   return (rmax - rad) * kexp * t1 / ((1.0 + t1) * (1.0 + t1));
 }
 
+// ---- generalized Born solvation (components/interactions/solvation) ----
+
+// Descreening contribution of atom j (scaled radius rho = 0.8 R_j) to the
+// integral psi_i of atom i (van der Waals radius ri) at distance r
+// (born.py:163-239), and its derivative d/dr.  Overlapping branch when
+// r < ri + rho and r + rho > ri, else the closed form of the
+// non-overlapping spheres.
+DXTB_HD void gb_psi_term(double r, double rho, double ri, double& t, double& dt) {
+  const double dpl = r + rho, dmi = r - rho;
+  if (r < ri + rho && dpl > ri) {
+    const double ir = 1.0 / r, ipl = 1.0 / dpl, iri2 = 1.0 / (ri * ri);
+    const double lg = log(dpl / ri);
+    const double a = 0.5 * dmi * (ipl - dpl * iri2) - lg;
+    const double da = 0.5 * (ipl - dpl * iri2) + 0.5 * dmi * (-ipl * ipl - iri2) - ipl;
+    t = 1.0 / ri - ipl + 0.5 * ir * a;
+    dt = ipl * ipl + 0.5 * ir * (da - a * ir);
+    return;
+  }
+  const double pr = dpl * dmi;  // r^2 - rho^2
+  t = rho / pr;
+  dt = -2.0 * r * rho / (pr * pr);
+  const double qq = dmi / dpl;
+  if (qq > 0.0) {
+    const double lq = log(qq);
+    t += 0.5 * lq / r;
+    dt += 0.5 * (2.0 * rho / (pr * r) - lq / (r * r));
+  }
+}
+
+// Born radius from psi (Onufriev-Bashford-Case, born.py:140-160) and dB/dpsi.
+DXTB_HD void gb_born_radius(double psi, double rvdw, double scale, double offset, const double obc[3], double& b,
+                            double& db) {
+  const double srvdw = rvdw - offset;
+  const double r1 = 1.0 / rvdw, s1 = 1.0 / srvdw;
+  const double p = 0.5 * psi * srvdw;
+  const double arg = p * (obc[0] + p * (obc[2] * p - obc[1]));
+  const double darg = 0.5 * srvdw * (obc[0] - 2.0 * obc[1] * p + 3.0 * obc[2] * p * p);
+  const double th = tanh(arg);
+  const double den = s1 - r1 * th;
+  b = scale / den;
+  db = scale * r1 * (1.0 - th * th) * darg / (den * den);
+}
+
+// Interaction kernels of the generalized Born model (alpb.py:94-141) with
+// ab = B_i B_j: value and derivatives by r and by ab.  kind 1: P16,
+// 1 / (r + s f^16), s = sqrt(ab), f = s / (s + 1.028 r / 16); kind 2: Still,
+// 1 / sqrt(r^2 + ab exp(-r^2 / (4 ab))).
+DXTB_HD void gb_kernel(int kind, double r, double ab, double& k, double& dkdr, double& dkdab) {
+  if (kind == 1) {
+    const double c = 1.028 / 16.0;
+    const double s = sqrt(ab);
+    const double f = s / (s + c * r);
+    double f16 = f * f;
+    f16 *= f16;
+    f16 *= f16;
+    f16 *= f16;  // f^16
+    k = 1.0 / (r + s * f16);
+    const double f17 = f16 * f;
+    dkdr = -k * k * (1.0 - 16.0 * c * f17);
+    dkdab = -k * k * (f16 + 16.0 * c * r * f17 / s) / (2.0 * s);
+  } else {
+    const double r2 = r * r;
+    const double e = exp(-0.25 * r2 / ab);
+    k = 1.0 / sqrt(r2 + ab * e);
+    const double k3 = k * k * k;
+    dkdr = -k3 * r * (1.0 - 0.25 * e);
+    dkdab = -0.5 * k3 * e * (1.0 + 0.25 * r2 / ab);
+  }
+}
+
 }  // namespace dxtb

@@ -39,6 +39,13 @@ KELVIN2AU = ELEMENTS["kb_hartree_per_kelvin"]
 COV_D3 = np.array([0.0] + [4.0 / 3.0 * r * AA2AU for r in ELEMENTS["covalent_2009"]])
 #: atomic radii of the H0 distance polynomial in Bohr, index = Z
 ATOMIC_RADII = np.array([0.0] + [r * AA2AU for r in ELEMENTS["atomic"]])
+#: D3 van der Waals radii (Angstrom, Z = 1..18) of the generalized Born
+#: solvation model (the VDW_D3 table the reference takes from tad-mctc);
+#: VDW_D3[Z] in Bohr, NaN where no radius is available
+_VDW_D3_AA = [1.09155, 0.86735, 1.74780, 1.54910, 1.60800, 1.45515, 1.31125, 1.24085, 1.14980, 1.06870,
+              1.85410, 1.74195, 2.00530, 1.89585, 1.75085, 1.65535, 1.55230, 1.45740]
+VDW_D3 = np.full(MAXZ + 1, np.nan)
+VDW_D3[1 : len(_VDW_D3_AA) + 1] = np.array(_VDW_D3_AA) * AA2AU
 
 _LVAL = {"s": 0, "p": 1, "d": 2, "f": 3, "g": 4}
 

@@ -24,12 +24,14 @@ vector; converged molecules are frozen while the rest keeps iterating.
 
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 
 import numpy as np
 import torch
 
 from . import _lib, param
+from . import solvation as gbsolv
 from .basis import BatchBasis
 from .integrals import multipole_forward
 from .params_dev import device_params
@@ -75,7 +77,8 @@ def _orthogonaliser(spad: torch.Tensor, chunk: int = 32) -> torch.Tensor:
 class GeometryState:
     """Everything that depends on the geometry but not on the density."""
 
-    def __init__(self, bb: BatchBasis, positions: torch.Tensor, charges=None, opts: SCFOptions | None = None):
+    def __init__(self, bb: BatchBasis, positions: torch.Tensor, charges=None, opts: SCFOptions | None = None,
+                 solvation=None):
         self.bb = bb
         self.opts = opts or SCFOptions()
         self.gfn = bb.gfn
@@ -101,6 +104,7 @@ class GeometryState:
                                  _lib.ptr(self.hfsh), _lib.ptr(self.erep), _lib.ptr(self.mrad), st)
         _lib.check(rc, "dxtb_setup")
         self.S, self.D, self.Q, self.H0 = multipole_forward(bb, self.pos, self.mp, fsh=self.fsh)
+        self._setup_solvation(solvation, st)
         # orthogonaliser X = L^-T with unit diagonal on padding
         spad = self.S.clone()
         pad = torch.arange(n, device=dev)[None, :] >= d["nao"][:, None]
@@ -131,9 +135,40 @@ class GeometryState:
         self.C = None  # previous orbitals (warm start of the Jacobi solver)
         self._jwork = None  # Jacobi workspace (a subset keeps the larger buffer)
 
+    def _setup_solvation(self, solvation, stream) -> None:
+        """Generalized Born solvation per molecule (keps = 0: none): Born
+        radii and dB/dpsi, and the ALPB shape descriptor."""
+        bb, d = self.bb, self.bb.dev
+        B, na = bb.nbatch, bb.nat_max
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        gbs = gbsolv.per_molecule(solvation, B)
+        self.any_gb = any(g is not None for g in gbs)
+        self.gb_par = gbsolv.gb_params(gbs)
+        self.gb_keps = torch.as_tensor([g.keps if g else 0.0 for g in gbs], **f64)
+        self.gb_kind = torch.as_tensor([gbsolv.KERNELS[g.kernel] if g else 0 for g in gbs], dtype=torch.int32,
+                                       device=self.dev)
+        self.gb_alpbet = torch.as_tensor([g.alpbet if g else 0.0 for g in gbs], **f64)
+        self._alpb = any(g is not None and g.alpb for g in gbs)
+        self.born = torch.ones((B, na), **f64)
+        self.dborn = torch.zeros((B, na), **f64)
+        self.gb_adet = torch.zeros(B, **f64)
+        if not self.any_gb:
+            return
+        gbsolv.check_elements(bb.numbers, gbs)
+        rc = self.lib.dxtb_gb_born(B, na, _lib.ptr(d["nat"]), _lib.ptr(d["numbers_i"]), _lib.ptr(self.pos),
+                                   _lib.ptr(self.gb_keps), _lib.ptr(gbsolv.vdw_device(self.dev)),
+                                   ctypes.byref(self.gb_par), _lib.ptr(self.born), _lib.ptr(self.dborn), stream)
+        _lib.check(rc, "dxtb_gb_born")
+        if self._alpb:
+            self.gb_adet = torch.nan_to_num(self._adet(self.pos))
+
+    def _adet(self, pos):
+        z = self.bb.dev["numbers_i"].long()
+        return gbsolv.adet(pos, gbsolv.vdw_device(self.dev)[z], z != 0)
+
     # ---- culling: a state restricted to some molecules of the batch --------
     _PER_MOLECULE = ("pos", "cn", "fsh", "dfsh", "hfsh", "erep", "mrad", "S", "D", "Q", "H0", "X", "nab", "n0",
-                     "valid_sh", "C")
+                     "valid_sh", "C", "gb_keps", "gb_kind", "gb_alpbet", "gb_adet", "born", "dborn")
     _GLOBAL_DEV = ("ush_l", "ush_np", "ush_alpha", "ush_coeff", "item_mol", "item_pair")
 
     _BB_HOST = ("numbers", "nat", "nsh", "nao", "sh_atom", "sh_ush", "sh_ao", "sh_l", "ao_atom", "ao_shell")
@@ -205,6 +240,10 @@ class GeometryState:
         if self.C is None:
             self.C = self.X.clone()
         self.C.index_copy_(0, rows, other.X)
+        self.any_gb = self.any_gb or other.any_gb
+        self._alpb = self._alpb or other._alpb
+        if other.any_gb:
+            self.gb_par = other.gb_par
         for k, v in ob.dev.items():
             if k not in self._GLOBAL_DEV:
                 v.index_copy_(0, rows, nb.dev[k])
@@ -323,6 +362,16 @@ class GeometryState:
                                      _lib.ptr(qdp), _lib.ptr(qqp), _lib.ptr(vsh), _lib.ptr(vat), _lib.ptr(vdp),
                                      _lib.ptr(vqp), _lib.ptr(esh), _lib.ptr(eat), _lib.stream_ptr(self.dev))
         _lib.check(rc, "dxtb_potential")
+        if self.any_gb:
+            rc = self.lib.dxtb_gb_potential(B, na, _lib.ptr(d["nat"]), _lib.ptr(self.pos), _lib.ptr(self.gb_keps),
+                                            _lib.ptr(self.gb_kind), _lib.ptr(self.born), _lib.ptr(qat), _lib.ptr(vat),
+                                            _lib.ptr(eat), _lib.stream_ptr(self.dev))
+            _lib.check(rc, "dxtb_gb_potential")
+            if self._alpb:  # ALPB shape term: keps alpbet adet Q_tot on every atom
+                c = (self.gb_keps * self.gb_alpbet * self.gb_adet * qat.sum(-1)).unsqueeze(-1)
+                add = c * (d["numbers_i"] != 0)
+                vat = vat + add
+                eat = eat + 0.5 * qat * add
         # total shell potential (atom part spread to its shells)
         vtot = (vsh + torch.gather(vat, 1, d["sh_atom_l"])) * self.valid_sh
         return vtot, vdp, vqp, esh, eat
@@ -488,6 +537,18 @@ def forces(res: SCFResult) -> torch.Tensor:
                                    _lib.ptr(qdp), _lib.ptr(qqp), _lib.ptr(dEdh), _lib.ptr(dEdcn), _lib.ptr(grad),
                                    stream)
     _lib.check(rc, "dxtb_forces_classical")
+    if st.any_gb:
+        rc = lib.dxtb_gb_forces(bb.nbatch, bb.nat_max, _lib.ptr(d["nat"]), _lib.ptr(d["numbers_i"]), _lib.ptr(st.pos),
+                                _lib.ptr(st.gb_keps), _lib.ptr(st.gb_kind), _lib.ptr(gbsolv.vdw_device(st.dev)),
+                                ctypes.byref(st.gb_par), _lib.ptr(st.born), _lib.ptr(st.dborn), _lib.ptr(qat),
+                                _lib.ptr(grad), stream)
+        _lib.check(rc, "dxtb_gb_forces")
+        if st._alpb:  # d/dR of 1/2 keps alpbet adet Q_tot^2 (a 3x3 determinant per molecule)
+            with torch.enable_grad():
+                p = st.pos.detach().requires_grad_(True)
+                e = (0.5 * st.gb_keps * st.gb_alpbet * qat.sum(-1) ** 2 * torch.nan_to_num(st._adet(p))).sum()
+                (ga,) = torch.autograd.grad(e, p)
+            grad = grad + ga
     return -grad
 
 

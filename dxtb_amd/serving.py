@@ -52,6 +52,7 @@ class Request:
     forces: np.ndarray | None = None  # (nat, 3) -dE/dR, Eh/Bohr
     iterations: int = 0
     converged: bool = False
+    solvation: object = None  # GeneralizedBorn settings, or None (vacuum)
 
     @property
     def latency(self) -> float:
@@ -161,7 +162,8 @@ class SlotPool:
             P[i, : len(r.numbers)] = r.positions
         bb = build_batch(N, self.gfn, self.dev, self.dims)
         chg = np.array([r.charge for r in reqs], dtype=np.float64)
-        st = hscf.GeometryState(bb, torch.as_tensor(P, device=self.dev), charges=chg, opts=self.opts)
+        st = hscf.GeometryState(bb, torch.as_tensor(P, device=self.dev), charges=chg, opts=self.opts,
+                                solvation=[r.solvation for r in reqs])
         rt = torch.as_tensor(rows, device=self.dev)
         self.state.scatter(rt, st)
         for t in (self.x, self.y, self.xprev, self.yprev, self.hc, self.hd):
@@ -390,7 +392,7 @@ def plain_path(req: Request, gfn: str, device, opts: hscf.SCFOptions | None = No
     N = np.asarray(req.numbers, dtype=np.int64)[None]
     bb = bb_(N, gfn, dev)
     st = hscf.GeometryState(bb, torch.as_tensor(np.asarray(req.positions)[None], device=dev),
-                            charges=np.array([req.charge]), opts=opts)
+                            charges=np.array([req.charge]), opts=opts, solvation=[req.solvation])
     res = hscf.run_scf(st)
     f = hscf.forces(res)
     req.energy = float(res.energy[0])

@@ -105,6 +105,13 @@ def config_batch(config: int, count: int | None = None, nat: int | None = None):
     return nums, poss, chg
 
 
+def config3_solvation(count: int, fraction: float = 0.2) -> np.ndarray:
+    """Requests of the serving stream that enable implicit solvation (GBSA,
+    eps = 78.4): a seeded fraction, drawn independently of the molecules."""
+    rng = np.random.default_rng(seed(3) + 7)
+    return rng.random(count) < fraction
+
+
 def water_cluster(nmol: int = 667, density_gcc: float = 1.0, seed_: int = 20260004):
     """Rigid waters (O-H 1.81 Bohr, 104.5 deg) at random orientations in a cube."""
     rng = np.random.default_rng(seed_)

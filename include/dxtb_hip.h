@@ -177,6 +177,36 @@ int dxtb_forces_classical(const dxtb_basis_t* bas, const dxtb_params_t* par, con
                           const double* qdp, const double* qqp, double* dEdh, double* dEdcn, double* grad,
                           hipStream_t stream);
 
+/* ---- generalized Born implicit solvation ------------------------------
+ * Replaces the reference's GeneralizedBorn interaction
+ * (components/interactions/solvation/alpb.py:187-350: get_cache,
+ * get_monopole_atom_potential/energy) and get_born_radii (solvation/born.py:
+ * 52-239); the forces replace autograd through both.  Per molecule b:
+ * keps[b] = (1/eps - 1) / (1 + alpbet) (0: no solvation, molecule skipped),
+ * kind[b] = 1 (P16 kernel) or 2 (Still kernel).  rvdw_z: van der Waals radius
+ * by atomic number (Bohr, device).  nat_max <= 1024. */
+typedef struct {
+  double descreening; /* 0.8 */
+  double cutoff;      /* 66 Bohr */
+  double born_scale;  /* 1 */
+  double born_offset; /* 0 */
+  double obc[3];      /* 1.0, 0.8, 4.85 */
+} dxtb_gb_params_t;
+
+/* Born radii born (B, nat_max) and dB/dpsi dborn (B, nat_max). */
+int dxtb_gb_born(int nbatch, int nat_max, const int* nat, const int* numbers, const double* pos, const double* keps,
+                 const double* rvdw_z, const dxtb_gb_params_t* par, double* born, double* dborn, hipStream_t stream);
+
+/* Adds V_i = keps sum_j K(r_ij, B_i B_j) q_j to vat and q_i V_i / 2 to eat. */
+int dxtb_gb_potential(int nbatch, int nat_max, const int* nat, const double* pos, const double* keps, const int* kind,
+                      const double* born, const double* qat, double* vat, double* eat, hipStream_t stream);
+
+/* Adds dE_GB/dR at fixed charges (direct pair terms and the Born-radius
+ * chain) to grad (B, nat_max, 3). */
+int dxtb_gb_forces(int nbatch, int nat_max, const int* nat, const int* numbers, const double* pos, const double* keps,
+                   const int* kind, const double* rvdw_z, const dxtb_gb_params_t* par, const double* born,
+                   const double* dborn, const double* qat, double* grad, hipStream_t stream);
+
 /* Largest matrix order handled by dxtb_jacobi_eigh. */
 int dxtb_jacobi_max_n(void);
 

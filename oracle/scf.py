@@ -37,11 +37,15 @@ from __future__ import annotations
 import torch
 
 from . import coulomb, hamiltonian, integrals, params
+from . import solvation as gbsolv
 from .basis import Basis, build
 
 
 class System:
-    def __init__(self, numbers, positions: torch.Tensor, gfn: str, charge: float = 0.0, etemp: float = 300.0):
+    def __init__(self, numbers, positions: torch.Tensor, gfn: str, charge: float = 0.0, etemp: float = 300.0,
+                 solvation: dict | None = None):
+        # solvation: generalized Born, {"eps": dielectric constant, "alpb": bool, "kernel": "p16" | "still"}
+        self.solvation = solvation
         self.numbers = [int(z) for z in numbers if int(z) != 0]
         self.gfn = gfn
         self.positions = positions
@@ -85,6 +89,10 @@ class System:
             out["q"] = q
             out["aes"] = coulomb.aes2_cache(b, positions)
         out["erep"] = coulomb.repulsion_energy(self.numbers, positions, self.gfn)
+        if self.solvation is not None:
+            sv = self.solvation
+            out["gb"] = gbsolv.gb_matrix(self.numbers, positions, sv["eps"], alpb=sv.get("alpb", False),
+                                            kernel=sv.get("kernel", "p16"))
         return out
 
     # ---- SCF maps ----------------------------------------------------------
@@ -118,6 +126,8 @@ class System:
         if self.is_gfn2:
             va, vdp, vqp = coulomb.aes2_potential(m["aes"], qat, qdp, qqp)
             vat = vat + va
+        if "gb" in m:
+            vat = vat + m["gb"] @ qat
         vsh = vsh + vat[self.sh_atom]
         return vsh[self.ao_shell], vdp, vqp
 
@@ -188,6 +198,8 @@ class System:
             e = e + (m["hd"] * qat**3 / 3.0).sum()
         if self.is_gfn2:
             e = e + coulomb.aes2_energy(m["aes"], qat, qdp, qqp).sum()
+        if "gb" in m:
+            e = e + 0.5 * qat @ m["gb"] @ qat
         return e
 
     def free_energy(self, occ):
@@ -241,9 +253,10 @@ class Result(dict):
     pass
 
 
-def singlepoint(numbers, positions, gfn, charge=0.0, tol=1e-10, maxiter=300, forces=False):
-    """Total energy (repulsion + electronic + -TS) and optionally forces."""
-    sysm = System(numbers, positions, gfn, charge)
+def singlepoint(numbers, positions, gfn, charge=0.0, tol=1e-10, maxiter=300, forces=False, solvation=None):
+    """Total energy (repulsion + electronic + -TS) and optionally forces;
+    ``solvation`` adds generalized Born implicit solvation (see System)."""
+    sysm = System(numbers, positions, gfn, charge, solvation=solvation)
     with torch.no_grad():
         m = sysm.build(positions)
         n = sysm.nao + (9 * sysm.nat if sysm.is_gfn2 else 0)

@@ -82,6 +82,46 @@ int main() {
       check("mrad", a, (mr(cn + h) - mr(cn - h)) / (2 * h), a);
     }
   }
+  // generalized Born: psi pair term (both branches), Born radius, kernels
+  for (int trial = 0; trial < 400; ++trial) {
+    const double rho = 0.8 * (2.0 + U(rng)), ri = 2.2 + U(rng);
+    const double r = 0.5 + 7.0 * (0.5 + 0.5 * U(rng));
+    double t, dt, tp, tm, d;
+    gb_psi_term(r, rho, ri, t, dt);
+    // skip points within h of a branch switch (the integrand is continuous there, its derivative is not)
+    if (fabs(r - (ri + rho)) < 1e-4 || fabs(r + rho - ri) < 1e-4 || fabs(r - rho) < 1e-3) continue;
+    gb_psi_term(r + h, rho, ri, tp, d);
+    gb_psi_term(r - h, rho, ri, tm, d);
+    check("gb psi dr", dt, (tp - tm) / (2 * h), dt);
+  }
+  {
+    const double obc[3] = {1.0, 0.8, 4.85};
+    for (int trial = 0; trial < 100; ++trial) {
+      const double psi = 0.4 * (0.5 + 0.5 * U(rng)), rv = 2.0 + U(rng);
+      double b, db, bp, bm, d;
+      gb_born_radius(psi, rv, 1.0, 0.0, obc, b, db);
+      gb_born_radius(psi + h, rv, 1.0, 0.0, obc, bp, d);
+      gb_born_radius(psi - h, rv, 1.0, 0.0, obc, bm, d);
+      check("gb born dpsi", db, (bp - bm) / (2 * h), db);
+    }
+  }
+  for (int kind = 1; kind <= 2; ++kind)
+    for (int trial = 0; trial < 200; ++trial) {
+      const double r = (trial % 10 == 0) ? 0.0 : 6.0 * (0.5 + 0.5 * U(rng));
+      const double ab = 4.0 + 8.0 * (0.5 + 0.5 * U(rng));
+      double k, dr, dab, kp, km, d1, d2;
+      gb_kernel(kind, r, ab, k, dr, dab);
+      if (r > 0.0) {
+        gb_kernel(kind, r + h, ab, kp, d1, d2);
+        gb_kernel(kind, r - h, ab, km, d1, d2);
+        check("gb kernel dr", dr, (kp - km) / (2 * h), dr);
+      } else {
+        check("gb kernel r=0", k, 1.0 / sqrt(ab), k);
+      }
+      gb_kernel(kind, r, ab + h, kp, d1, d2);
+      gb_kernel(kind, r, ab - h, km, d1, d2);
+      check("gb kernel dab", dab, (kp - km) / (2 * h), dab);
+    }
   printf("maxerr %.3e\n", maxerr);
   return maxerr < 1e-7 ? 0 : 1;
 }

@@ -13,6 +13,8 @@ plain data to ``tests/golden``:
   ``test/test_coulomb/test_aes2.py`` (AES2 energies and potentials),
   ``test/test_scf/samples.py`` (SCF electronic energies),
   ``test/test_classical/test_repulsion/samples.py`` (repulsion energies),
+  ``test/test_solvation/samples.py`` (Born radii, psi, generalized Born
+  energies for given charges, GFN1 solvation energies),
   ``test/test_singlepoint/refs/gfn2/*.txt`` (tblite GFN2 energies/gradients).
 
 The literals are evaluated with ``ast`` as data only.  Run:
@@ -143,6 +145,7 @@ def main(root: str = "/root/reference") -> None:
         "aes2_potential": function_locals(t / "test_coulomb" / "test_aes2.py", "test_potential"),
         "scf": module_dict(t / "test_scf" / "samples.py", "refs"),
         "repulsion": module_dict(t / "test_classical" / "test_repulsion" / "samples.py", "refs"),
+        "solvation": module_dict(t / "test_solvation" / "samples.py", "refs"),
     }
     refs = {}
     for f in sorted((t / "test_singlepoint" / "refs" / "gfn2").glob("*.txt")):

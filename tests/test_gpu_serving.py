@@ -23,24 +23,32 @@ from dxtb_amd import workloads
 pytestmark = pytest.mark.gpu
 
 
-def _requests(k=8, maxnat=30):
+def _requests(k=8, maxnat=30, solvated=()):
+    """Seeded serving requests; the ones at positions ``solvated`` use GBSA."""
+    from dxtb_amd.solvation import GeneralizedBorn
+
     nums, poss, chg = workloads.config_batch(3, count=60)
     sel = [i for i in range(len(nums)) if len(nums[i]) <= maxnat][:k]
-    return [sv.Request(i, nums[i], poss[i], float(chg[i])) for i in sel]
+    gb = GeneralizedBorn(None, 78.4, alpb=False)
+    return [sv.Request(i, nums[i], poss[i], float(chg[i]), solvation=gb if k_ in solvated else None)
+            for k_, i in enumerate(sel)]
 
 
 def test_alone_vs_mix(cuda):
-    reqs = _requests()
+    reqs = _requests(solvated=(1, 4))
     assert len({sv.pool_class(*sv.sizes(r.numbers, "gfn2")) for r in reqs}) >= 2
     print("case serving mix of", [len(r.numbers) for r in reqs], flush=True)
     eng = sv.Engine("gfn2", cuda, capacity={c: 2 for c in sv.POOL_NAO})
-    out = eng.run([sv.Request(r.rid, r.numbers, r.positions, r.charge) for r in reqs], realtime=False)
+    out = eng.run([sv.Request(r.rid, r.numbers, r.positions, r.charge, solvation=r.solvation) for r in reqs],
+                  realtime=False)
     assert len(out) == len(reqs)
     got = {r.rid: r for r in out}
     for r in reqs:
         m = got[r.rid]
-        alone = sv.Engine("gfn2", cuda).run([sv.Request(r.rid, r.numbers, r.positions, r.charge)], realtime=False)[0]
-        ref = sv.plain_path(sv.Request(r.rid, r.numbers, r.positions, r.charge), "gfn2", cuda)
+        alone = sv.Engine("gfn2", cuda).run([sv.Request(r.rid, r.numbers, r.positions, r.charge,
+                                                        solvation=r.solvation)], realtime=False)[0]
+        ref = sv.plain_path(sv.Request(r.rid, r.numbers, r.positions, r.charge, solvation=r.solvation), "gfn2",
+                            cuda)
         print(f"case {r.rid}: iterations mix {m.iterations} alone {alone.iterations} plain {ref.iterations}, "
               f"dE {m.energy - alone.energy:.2e} {m.energy - ref.energy:.2e}", flush=True)
         assert m.converged and alone.converged and ref.converged

@@ -54,3 +54,16 @@ def test_jacobi_arguments_without_gpu():
     args = (None,) * 5
     assert lib.dxtb_jacobi_eigh(0, 10, *args, 0, 10, 1e-14, None, None) == _lib.DXTB_EINVAL
     assert lib.dxtb_jacobi_eigh(1, lib.dxtb_jacobi_max_n() + 1, *args, 0, 10, 1e-14, None, None) == _lib.DXTB_EINVAL
+
+
+def test_ctypes_arity_matches_header():
+    """Every ctypes signature has as many arguments as its declaration in the
+    header (a missing or extra argument would shift every pointer after it)."""
+    text = "\n".join(p.read_text() for p in (ROOT / "include").glob("*.h"))
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    for name, args in _lib._SIGNATURES.items():
+        m = re.search(r"\b" + name + r"\s*\(([^)]*)\)\s*;", text, flags=re.S)
+        assert m, name
+        params = m.group(1).strip()
+        n = 0 if params in ("", "void") else params.count(",") + 1
+        assert n == len(args), f"{name}: header has {n} parameters, ctypes {len(args)}"
