@@ -38,8 +38,10 @@
 
 namespace {
 
-constexpr int JB = 256;                // threads per workgroup (both variants)
-constexpr int NWAVE = JB / 64;         // wavefronts per workgroup
+constexpr int JB = 256;                // threads per workgroup of the scalar variant
+constexpr int NWAVE = JB / 64;         // its wavefronts
+constexpr int JBB = 512;               // threads per workgroup of the blocked variant
+constexpr int NWB = JBB / 64;          // its wavefronts (one block pair or product item each)
 constexpr int JMAX_LDS = 128;          // largest matrix of the scalar variant
 constexpr int JSTRIDE = JMAX_LDS + 1;  // padded LDS row stride (doubles)
 constexpr int JMAX = 512;              // largest matrix overall
@@ -50,6 +52,7 @@ constexpr int SST = SB + 1;            // LDS stride of a subproblem
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
+template <int NW = NWAVE>
 __device__ __forceinline__ double block_max(double v, double* sh) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
@@ -59,7 +62,7 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
   __syncthreads();
   double r = sh[0];
 #pragma unroll
-  for (int k = 1; k < NWAVE; ++k) r = fmax(r, sh[k]);
+  for (int k = 1; k < NW; ++k) r = fmax(r, sh[k]);
   return r;
 }
 
@@ -240,9 +243,9 @@ struct WaveLds {
 constexpr int KMAX = JMAX / SB;  // max block pairs
 
 struct BlockedLds {
-  WaveLds w[NWAVE];
+  WaveLds w[NWB];
   int rot[KMAX];  // pair k rotated anything in this outer step
-  double red[NWAVE];
+  double red[NWB];
 };
 
 // global row/column of local index r (0..31) of the block pair (bi, bj)
@@ -388,7 +391,7 @@ __device__ __forceinline__ void block_pair(int k, int s, int nb, int& bi, int& b
   bj = max(a, c);
 }
 
-__global__ __launch_bounds__(JB, 2) void k_jacobi_blocked(int nmax, const int* __restrict__ nsize, double* __restrict__ Ag,
+__global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const int* __restrict__ nsize, double* __restrict__ Ag,
                                                        double* __restrict__ VT, double* __restrict__ evals,
                                                        double* __restrict__ work, int max_sweeps, double tol,
                                                        int* __restrict__ sweeps_out) {
@@ -403,12 +406,12 @@ __global__ __launch_bounds__(JB, 2) void k_jacobi_blocked(int nmax, const int* _
   double* eb = evals + (size_t)b * nmax;
   double* Qw = work + (size_t)b * npad * SB;  // K rotations of 32x32
 
-  for (int e = threadIdx.x; e < nmax * nmax; e += JB) {
+  for (int e = threadIdx.x; e < nmax * nmax; e += JBB) {
     const int i = e / nmax, j = e - i * nmax;
     Vb[e] = (i == j) ? 1.0 : 0.0;
   }
   if (n == 0) {
-    for (int i = threadIdx.x; i < nmax; i += JB) eb[i] = 0.0;
+    for (int i = threadIdx.x; i < nmax; i += JBB) eb[i] = 0.0;
     if (threadIdx.x == 0 && sweeps_out != nullptr) sweeps_out[b] = 0;
     return;
   }
@@ -424,15 +427,15 @@ __global__ __launch_bounds__(JB, 2) void k_jacobi_blocked(int nmax, const int* _
     // max |offdiag| and max |diag|, eight independent loads in flight per thread
     double off = 0.0, amax = 0.0;
     const int nn = n * n;
-    for (int e0 = threadIdx.x; e0 < nn; e0 += 8 * JB) {
+    for (int e0 = threadIdx.x; e0 < nn; e0 += 8 * JBB) {
       double v[8];
       bool dg[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const int e = min(e0 + u * JB, nn - 1);
+        const int e = min(e0 + u * JBB, nn - 1);
         const int i = e / n, j = e - i * n;
         const double x = A[(size_t)i * nmax + j];
-        v[u] = fabs(x) * mask01(e0 + u * JB < nn);
+        v[u] = fabs(x) * mask01(e0 + u * JBB < nn);
         dg[u] = i == j;
       }
 #pragma unroll
@@ -441,14 +444,14 @@ __global__ __launch_bounds__(JB, 2) void k_jacobi_blocked(int nmax, const int* _
         amax = fmax(amax, dg[u] ? v[u] : 0.0);
       }
     }
-    off = block_max(off, L.red);  // ends with a barrier: V init and last step visible
-    amax = block_max(amax, L.red);
+    off = block_max<NWB>(off, L.red);  // ends with a barrier: V init and last step visible
+    amax = block_max<NWB>(amax, L.red);
     const double thr = fmax(1e-300, tol * fmax(amax, 1.0));
     if (stop_sweeps(off, off_prev, thr, sweep)) break;
     off_prev = off;
     for (int s = 0; s < nb - 1; ++s) {
       // ---- inner phase: one wavefront per block pair
-      for (int k = wv; k < npairs; k += NWAVE) {
+      for (int k = wv; k < npairs; k += NWB) {
         int bi, bj;
         block_pair(k, s, nb, bi, bj);
         double v[16];
@@ -488,7 +491,7 @@ __global__ __launch_bounds__(JB, 2) void k_jacobi_blocked(int nmax, const int* _
       const int ntile = npairs * (npairs - 1) / 2;
       const int npanel = (n + SB - 1) / SB;
       const int nwork = ntile + npairs * npanel;
-      for (int item = wv; item < nwork; item += NWAVE) {
+      for (int item = wv; item < nwork; item += NWB) {
         if (item < ntile) {
           int k = 0, rem = item;  // tile index -> (k, k2), k < k2
           while (rem >= npairs - 1 - k) {
@@ -584,7 +587,7 @@ __global__ __launch_bounds__(JB, 2) void k_jacobi_blocked(int nmax, const int* _
       __syncthreads();
     }
   }
-  for (int i = threadIdx.x; i < nmax; i += JB) eb[i] = (i < n) ? A[(size_t)i * nmax + i] : 0.0;
+  for (int i = threadIdx.x; i < nmax; i += JBB) eb[i] = (i < n) ? A[(size_t)i * nmax + i] : 0.0;
   if (threadIdx.x == 0 && sweeps_out != nullptr) sweeps_out[b] = sweep;
 }
 
@@ -622,7 +625,7 @@ extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* 
     hipLaunchKernelGGL(k_jacobi_scalar, dim3((unsigned)nbatch), dim3(JB), scalar_shmem(), st, nmax, nsize, A, VT,
                        evals, max_sweeps, tol, sweeps);
   else
-    hipLaunchKernelGGL(k_jacobi_blocked, dim3((unsigned)nbatch), dim3(JB), sizeof(BlockedLds), st, nmax, nsize, A,
+    hipLaunchKernelGGL(k_jacobi_blocked, dim3((unsigned)nbatch), dim3(JBB), sizeof(BlockedLds), st, nmax, nsize, A,
                        VT, evals, work, max_sweeps, tol, sweeps);
   return (int)hipGetLastError();
 }
