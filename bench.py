@@ -10,6 +10,8 @@ Workloads:
   serving    -- serving stream: log-normal molecule sizes (median 40 atoms),
                 Poisson arrivals at three offered loads; molecules/s and
                 p50/p99 latency, against the plain one-molecule-per-call path.
+  eigen      -- eigensolver microbenchmark: one sweep of the blocked Jacobi
+                kernel on 256 random symmetric matrices of order 240.
   integrals  -- integral-driver microbenchmark: overlap + dipole + quadrupole
                 and their d/dR VJP for a padded batch of 512 GFN2 molecules of
                 60 atoms (H:C:N:O:S = .45:.3:.1:.1:.05); rate in shell pairs/s.
@@ -179,42 +181,139 @@ def jacobi_flops_per_sweep(n: int, variant: int) -> float:
     return (nb - 1) * (ntile * 2 * gemm + npairs * npanel * gemm) + npairs * steps_inner * inner
 
 
+# measured HBM traffic of k_jacobi_blocked / its algorithmic bytes: rocprofv3
+# FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the same counters on a
+# copy of known size (eigen workload, 256 x order 240)
+JACOBI_TRAFFIC_RATIO = 1.087
+
+
+def jacobi_bytes(n: int, variant: int) -> float:
+    """Algorithmic HBM bytes of one sweep: blocked variant, A and VT read and
+    written once per outer step; scalar variant (A in LDS), the two VT rows of
+    every rotation read and written once per step."""
+    if variant == 2:
+        return jacobi_bytes_per_sweep(n)
+    m = n + (n & 1)
+    return (m - 1) * 2.0 * n * n * 8
+
+
 def jacobi_roofline(records) -> dict:
-    """Achieved FP64 rate of the Jacobi launches recorded in the timed region
-    (HIP events on the launch stream; flops from the sweeps each matrix ran)."""
+    """Achieved HBM rate of the Jacobi launches recorded in the timed region
+    (HIP events on the launch stream; algorithmic bytes and flops from the
+    sweeps each matrix ran).  At full occupancy the blocked kernel is bound by
+    HBM traffic (see its microbenchmark); with several pool streams in flight
+    an event's time includes time shared with other kernels, so the rate is a
+    lower bound."""
     if not records:
         return None
     t_ms = []
     flops = 0.0
+    nbytes = 0.0
     nsweep, nmat = 0, 0
-    by_var = {1: [0.0, 0.0], 2: [0.0, 0.0]}
     for e0, e1, n, nao, sw, variant in records:
-        ms = e0.elapsed_time(e1)
-        t_ms.append(ms)
+        t_ms.append(e0.elapsed_time(e1))
         sweeps = sw.cpu().numpy()
         nsweep += int(sweeps.sum())
         nmat += len(sweeps)
-        f = float(sum(int(sweeps[i]) * jacobi_flops_per_sweep(int(nao[i]), variant) for i in range(len(sweeps))))
-        flops += f
-        by_var[variant][0] += f
-        by_var[variant][1] += ms
+        for i in range(len(sweeps)):
+            k = int(sweeps[i])
+            if k:
+                flops += k * jacobi_flops_per_sweep(int(nao[i]), variant)
+                nbytes += k * jacobi_bytes(int(nao[i]), variant)
     total_ms = float(np.sum(t_ms))
-    achieved = flops / (total_ms * 1e-3) / 1e12
+    achieved = nbytes / (total_ms * 1e-3) / 1e9
     return {
         "kernel": "k_jacobi_blocked / k_jacobi_scalar (dxtb_jacobi_eigh)",
-        "bound": "mfma",
+        "bound": "hbm",
         "achieved": achieved,
-        "peak": FP64_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": achieved / FP64_PEAK_TFLOPS,
-        "traffic": None,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": JACOBI_TRAFFIC_RATIO * nbytes / len(t_ms),
+        "algorithmic_bytes_per_launch": nbytes / len(t_ms),
+        "fp64_tflops": flops / (total_ms * 1e-3) / 1e12,
+        "fp64_peak_tflops": FP64_PEAK_TFLOPS,
         "launches": len(t_ms),
         "avg_launch_ms": total_ms / len(t_ms),
         "total_kernel_ms": total_ms,
         "mean_sweeps": nsweep / max(nmat, 1),
-        "blocked_tflops": by_var[2][0] / max(by_var[2][1], 1e-9) / 1e9,
-        "scalar_tflops": by_var[1][0] / max(by_var[1][1], 1e-9) / 1e9,
     }
+
+
+def jacobi_bytes_per_sweep(n: int) -> float:
+    """HBM bytes one sweep of the blocked Jacobi variant moves for an order-n
+    matrix: every outer step reads and writes A and the eigenvector rows VT
+    once (4 n^2 doubles; the 32x32 rotations are L2-resident and small)."""
+    nb = 2 * (-(-n // 32))
+    return (nb - 1) * 4.0 * n * n * 8
+
+
+def bench_eigen(args, dev):
+    """Batched eigensolver microbenchmark: the blocked Jacobi kernel on
+    nmol random symmetric matrices of order n (the size class of a 60-80 atom
+    GFN2 molecule), one sweep per launch so every launch does the same work;
+    a device copy of known size runs first (calibration of the HBM
+    counters)."""
+    from dxtb_amd import _lib
+
+    lib = _lib.load()
+    B, n = args.nmol, args.eig_n
+    g = torch.Generator(device="cpu").manual_seed(11)
+    M = torch.randn((B, n, n), generator=g, dtype=torch.float64)
+    A0 = (0.5 * (M + M.transpose(1, 2))).to(dev)
+    A = torch.empty_like(A0)
+    VT = torch.empty_like(A0)
+    ev = torch.empty((B, n), dtype=torch.float64, device=dev)
+    ns = torch.full((B,), n, dtype=torch.int32, device=dev)
+    sw = torch.zeros(B, dtype=torch.int32, device=dev)
+    work = torch.empty(lib.dxtb_jacobi_work_size(B, n) // 8 + 1, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    # calibration copy: 512 MiB read + 512 MiB written
+    src = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize(dev)
+
+    def launch():
+        A.copy_(A0)  # the blocked variant works in place
+        rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work), 2, 1,
+                                  0.0, _lib.ptr(sw), _lib.stream_ptr(dev))
+        _lib.check(rc, "dxtb_jacobi_eigh")
+
+    for _ in range(args.warmup):
+        launch()
+    torch.cuda.synchronize(dev)
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        A.copy_(A0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work), 2, 1,
+                                  0.0, _lib.ptr(sw), _lib.stream_ptr(dev))
+        e1.record(stream)
+        _lib.check(rc, "dxtb_jacobi_eigh")
+        e1.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    wall = (time.perf_counter() - t0) / args.steps
+    assert int(sw.min()) == 1 and int(sw.max()) == 1
+    t = float(np.mean(ms)) * 1e-3
+    nbytes = B * jacobi_bytes_per_sweep(n)
+    flops = B * jacobi_flops_per_sweep(n, 2)
+    res = {
+        "metric": "batched Jacobi sweeps/s (blocked variant, FP64)",
+        "value": B / t,
+        "unit": "matrix-sweeps/s",
+        "ms_per_step": wall * 1e3,
+        "dtype": "f64",
+        "config": {"workload": f"eigensolver microbenchmark: {B} random symmetric matrices of order {n}, one sweep "
+                               "per launch", "kernel_ms": t * 1e3,
+                   "fp64_tflops": flops / t / 1e12, "fp64_peak_tflops": FP64_PEAK_TFLOPS},
+        "roofline": {"kernel": "k_jacobi_blocked", "bound": "hbm", "achieved": nbytes / t / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": nbytes / t / 1e9 / HBM_PEAK_GBS,
+                     "traffic": JACOBI_TRAFFIC_RATIO * nbytes, "bytes_per_launch": nbytes},
+    }
+    return res, None
 
 
 def _progress(label):
@@ -401,7 +500,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="gfn2-batch", choices=["gfn2-batch", "serving", "integrals"])
+    ap.add_argument("--workload", default="gfn2-batch", choices=["gfn2-batch", "serving", "integrals", "eigen"])
+    ap.add_argument("--eig-n", type=int, default=240)
     ap.add_argument("--nmol", type=int, default=None)
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -416,6 +516,10 @@ def main():
     if args.workload == "integrals":
         args.nmol = args.nmol or 512
         res, extra = bench_integrals(args, dev)
+    elif args.workload == "eigen":
+        args.nmol = args.nmol or 256
+        args.no_cpu_baseline = True
+        res, extra = bench_eigen(args, dev)
     elif args.workload == "serving":
         args.nmol = args.nmol or 1200
         res, extra = bench_serving(args, dev)
