@@ -374,9 +374,10 @@ def bench_gfn2_batch(args, dev):
     wall = float(np.mean(walls))
     lat = np.array([r.latency for r in out])
     its = np.array([r.iterations for r in out])
+    nconv = int(sum(r.converged for r in out))
     res = {
         "metric": "GFN2 energy+forces molecules/s, p50/p99 latency at |dE|<1e-8 Eh",
-        "value": len(out) / wall,
+        "value": nconv / wall,  # molecules that met the SCF bar (the others stop at the iteration limit)
         "unit": "molecules/s",
         "ms_per_step": wall * 1e3,
         "dtype": "f64",
@@ -386,8 +387,9 @@ def bench_gfn2_batch(args, dev):
                    "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
                    "scf_iterations_mean": float(its.mean()),
                    "scf_iterations_max": int(its.max()),
-                   "converged": int(sum(r.converged for r in out)),
-                   "molecules": len(out)},
+                   "converged": nconv,
+                   "molecules": len(out),
+                   "all_molecules_per_s": len(out) / wall},
         "roofline": roof,
     }
     return res, (nums, poss, chg, "config-2")

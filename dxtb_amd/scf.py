@@ -131,6 +131,19 @@ class GeometryState:
         self.n0 = torch.as_tensor(n0, **f64).contiguous()
         self.kt = self.opts.etemp * param.KELVIN2AU
         self.nvec = ns + (9 * na if self.mp else 0)
+        # inner-product weights of the potential vector: the reference iterates
+        # the orbital-resolved monopole potential (one entry per AO, a shell's
+        # value repeated 2l+1 times; components/interactions/container.py:
+        # 110-141), so in the shell-resolved vector every dot product and norm
+        # weights a shell by 2l+1; the atomic dipole/quadrupole entries weigh 1
+        w = np.zeros((B, self.nvec))
+        for b in range(B):
+            k = int(bb.nsh[b])
+            w[b, :k] = 2 * bb.sh_l[b, :k] + 1
+            if self.mp:
+                w[b, ns : ns + 3 * int(bb.nat[b])] = 1.0
+                w[b, ns + 3 * na : ns + 3 * na + 6 * int(bb.nat[b])] = 1.0
+        self.wvec = torch.as_tensor(w, **f64)
         self.valid_sh = (torch.arange(ns, device=dev)[None, :] < d["nsh"][:, None]).double()
         self.C = None  # previous orbitals (warm start of the Jacobi solver)
         self._jwork = None  # Jacobi workspace (a subset keeps the larger buffer)
@@ -168,7 +181,7 @@ class GeometryState:
 
     # ---- culling: a state restricted to some molecules of the batch --------
     _PER_MOLECULE = ("pos", "cn", "fsh", "dfsh", "hfsh", "erep", "mrad", "S", "D", "Q", "H0", "X", "nab", "n0",
-                     "valid_sh", "C", "gb_keps", "gb_kind", "gb_alpbet", "gb_adet", "born", "dborn")
+                     "valid_sh", "C", "gb_keps", "gb_kind", "gb_alpbet", "gb_adet", "born", "dborn", "wvec")
     _GLOBAL_DEV = ("ush_l", "ush_np", "ush_alpha", "ush_coeff", "item_mol", "item_pair")
 
     _BB_HOST = ("numbers", "nat", "nsh", "nao", "sh_atom", "sh_ush", "sh_ao", "sh_l", "ao_atom", "ao_shell")
@@ -389,15 +402,24 @@ class GeometryState:
         return self.pack(vtot, vdp, vqp)
 
 
+def wnorm(v: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
+    """Norm of the potential vectors in the reference's (orbital-resolved)
+    space: sqrt(sum_i w_i v_i^2)."""
+    return torch.sqrt((w * v * v).sum(-1))
+
+
 class BatchedBroyden:
     """Per-molecule xitorch-style broyden1 (good Broyden, rank-one updates of the
-    inverse Jacobian G = -alpha I + sum_k c_k d_k^T).  The update vectors live
-    in preallocated (B, H, L) buffers so that a product with G is two batched
-    GEMVs whatever the history length."""
+    inverse Jacobian G = -alpha I + sum_k c_k <d_k, .>), with the inner product
+    <u, v> = sum_i w_i u_i v_i of the orbital-resolved potential vector (see
+    GeometryState.wvec): the iterates are those of broyden1 on the reference's
+    vector.  The update vectors live in preallocated (B, H, L) buffers so that
+    a product with G is two batched GEMVs whatever the history length."""
 
-    def __init__(self, alpha: float, damp: float, B: int, L: int, H: int, device):
+    def __init__(self, alpha: float, damp: float, B: int, L: int, H: int, device, w: torch.Tensor):
         self.g0 = -alpha
         self.damp = damp
+        self.w = w
         self.C = torch.zeros((B, H, L), dtype=torch.float64, device=device)
         self.D = torch.zeros((B, H, L), dtype=torch.float64, device=device)
         self.k = 0
@@ -406,7 +428,7 @@ class BatchedBroyden:
         r = self.g0 * v
         if self.k == 0:
             return r
-        coef = torch.bmm(Bm[:, : self.k], v.unsqueeze(-1))  # (B, k, 1)
+        coef = torch.bmm(Bm[:, : self.k], (self.w * v).unsqueeze(-1))  # (B, k, 1)
         return r + torch.bmm(A[:, : self.k].transpose(1, 2), coef).squeeze(-1)
 
     def mv(self, v):
@@ -420,7 +442,7 @@ class BatchedBroyden:
             # xitorch "restart" reduction: forget the history
             self.k = 0
         v = self.rmv(ddx)
-        den = (ddy * v).sum(-1, keepdim=True)
+        den = (self.w * ddy * v).sum(-1, keepdim=True)
         ok = active.unsqueeze(-1) & (den != 0)
         self.C[:, self.k] = torch.where(ok, ddx - self.mv(ddy), 0.0)
         self.D[:, self.k] = torch.where(ok, v / torch.where(den == 0, 1.0, den), 0.0)
@@ -456,18 +478,18 @@ def run_scf(state: GeometryState, cull: float = 0.75) -> SCFResult:
     cur = state
     x = x_full.clone()
     y = x - cur.g(x)
-    mix = BatchedBroyden(o.alpha, o.damp, B, state.nvec, o.maxiter + 1, dev)
+    mix = BatchedBroyden(o.alpha, o.damp, B, state.nvec, o.maxiter + 1, dev, state.wvec)
     active = torch.ones(B, dtype=torch.bool, device=dev)
     iters = np.zeros(B, dtype=np.int64)
     conv = np.zeros(B, dtype=bool)
     xprev, yprev = x, y
     for it in range(o.maxiter):
         dx = -mix.mv(y)
-        dxn = dx.norm(dim=-1)
+        dxn = wnorm(dx, cur.wvec)
         xnew = torch.where(active.unsqueeze(-1), x + o.damp * dx, x)
         ynew = xnew - cur.g(xnew)
         ynew = torch.where(active.unsqueeze(-1), ynew, y)
-        yn = ynew.norm(dim=-1)
+        yn = wnorm(ynew, cur.wvec)
         mix.update(xnew - xprev, ynew - yprev, active)
         xprev, yprev = xnew, ynew
         done = active & (dxn < o.x_tol) & (yn < o.f_tol)
@@ -489,7 +511,7 @@ def run_scf(state: GeometryState, cull: float = 0.75) -> SCFResult:
             idx = idx[keep]
             x, y, xprev, yprev = x[keep], y[keep], xprev[keep], yprev[keep]
             active = active[keep]
-            mix.C, mix.D = mix.C[keep], mix.D[keep]
+            mix.C, mix.D, mix.w = mix.C[keep], mix.D[keep], cur.wvec
     if bool(active.any()):
         x_full[idx[active]] = x[active]
     return finalize(state, x_full, iters, conv)

@@ -177,11 +177,13 @@ class SlotPool:
             self.req[row] = r
 
     # ---- one SCF iteration of every occupied slot --------------------------
-    def _mv(self, A, B, v, kmax):
+    def _mv(self, A, B, v, kmax, w):
+        """(g0 + sum_k a_k <b_k, .>_w) v with the weighted inner product of
+        the orbital-resolved potential vector (scf.BatchedBroyden)."""
         r = -self.opts.alpha * v
         if kmax == 0:
             return r
-        coef = torch.bmm(B[:, :kmax], v.unsqueeze(-1))
+        coef = torch.bmm(B[:, :kmax], (w * v).unsqueeze(-1))
         return r + torch.bmm(A[:, :kmax].transpose(1, 2), coef).squeeze(-1)
 
     def launch(self):
@@ -205,17 +207,18 @@ class SlotPool:
         x, y = self.x[:hi], self.y[:hi]
         xprev, yprev = self.xprev[:hi], self.yprev[:hi]
         hc, hd = self.hc[:hi], self.hd[:hi]
-        dx = -self._mv(hc, hd, y, kmax)
-        dxn = dx.norm(dim=-1)
+        w = win.wvec
+        dx = -self._mv(hc, hd, y, kmax, w)
+        dxn = hscf.wnorm(dx, w)
         xnew = torch.where(started.unsqueeze(-1), x + o.damp * dx, x)
         ynew = torch.where(occ.unsqueeze(-1), xnew - win.g(xnew), y)
-        yn = ynew.norm(dim=-1)
+        yn = hscf.wnorm(ynew, w)
         # Broyden update of the started slots at their own history index
         ddx, ddy = xnew - xprev, ynew - yprev
-        v = self._mv(hd, hc, ddx, kmax)
-        den = (ddy * v).sum(-1, keepdim=True)
+        v = self._mv(hd, hc, ddx, kmax, w)
+        den = (w * ddy * v).sum(-1, keepdim=True)
         ok = started.unsqueeze(-1) & (den != 0)
-        cnew = torch.where(ok, ddx - self._mv(hc, hd, ddy, kmax), 0.0)
+        cnew = torch.where(ok, ddx - self._mv(hc, hd, ddy, kmax, w), 0.0)
         dnew = torch.where(ok, v / torch.where(den == 0, 1.0, den), 0.0)
         if len(srows):
             hc[ri, ki] = cnew[ri]
