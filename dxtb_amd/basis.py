@@ -147,6 +147,17 @@ class BatchBasis:
         return int(self.numbers.shape[0])
 
 
+def to_device(a, dev, dtype: torch.dtype | None = None) -> torch.Tensor:
+    """Host array to the device through pinned memory without waiting (a copy
+    from pageable memory can wait for every stream of the device)."""
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    if torch.device(dev).type != "cuda":
+        return t.to(dev)
+    return t.pin_memory().to(dev, non_blocking=True)
+
+
 def build_batch(numbers, gfn: str, device: torch.device | str, dims: tuple[int, int, int] | None = None
                 ) -> BatchBasis:
     """Shell maps and ordered shell-pair lists for a (padded) batch.
@@ -229,7 +240,7 @@ def build_batch(numbers, gfn: str, device: torch.device | str, dims: tuple[int, 
 
     dev = torch.device(device)
     td = table.device(dev)
-    t = lambda x: torch.as_tensor(np.ascontiguousarray(x), device=dev)  # noqa: E731
+    t = lambda x: to_device(x, dev)  # noqa: E731
     d = {
         "sh_atom": t(sh_atom),
         "sh_ush": t(sh_ush),

@@ -32,7 +32,7 @@ import torch
 
 from . import _lib, param
 from . import solvation as gbsolv
-from .basis import BatchBasis
+from .basis import BatchBasis, to_device
 from .integrals import multipole_forward
 from .params_dev import device_params
 
@@ -127,8 +127,8 @@ class GeometryState:
         nuhf = np.remainder(nint, 2)
         diff = np.minimum(nuhf, nel)
         nb = (nel - diff) / 2.0
-        self.nab = torch.as_tensor(np.stack([nb + diff, nb], 1), **f64).contiguous()
-        self.n0 = torch.as_tensor(n0, **f64).contiguous()
+        self.nab = to_device(np.stack([nb + diff, nb], 1).astype(np.float64), dev)
+        self.n0 = to_device(n0.astype(np.float64), dev)
         self.kt = self.opts.etemp * param.KELVIN2AU
         self.nvec = ns + (9 * na if self.mp else 0)
         # inner-product weights of the potential vector: the reference iterates
@@ -143,7 +143,7 @@ class GeometryState:
             if self.mp:
                 w[b, ns : ns + 3 * int(bb.nat[b])] = 1.0
                 w[b, ns + 3 * na : ns + 3 * na + 6 * int(bb.nat[b])] = 1.0
-        self.wvec = torch.as_tensor(w, **f64)
+        self.wvec = to_device(w, dev)
         self.valid_sh = (torch.arange(ns, device=dev)[None, :] < d["nsh"][:, None]).double()
         self.C = None  # previous orbitals (warm start of the Jacobi solver)
         self._jwork = None  # Jacobi workspace (a subset keeps the larger buffer)
@@ -157,10 +157,10 @@ class GeometryState:
         gbs = gbsolv.per_molecule(solvation, B)
         self.any_gb = any(g is not None for g in gbs)
         self.gb_par = gbsolv.gb_params(gbs)
-        self.gb_keps = torch.as_tensor([g.keps if g else 0.0 for g in gbs], **f64)
-        self.gb_kind = torch.as_tensor([gbsolv.KERNELS[g.kernel] if g else 0 for g in gbs], dtype=torch.int32,
-                                       device=self.dev)
-        self.gb_alpbet = torch.as_tensor([g.alpbet if g else 0.0 for g in gbs], **f64)
+        self.gb_keps = to_device(np.array([g.keps if g else 0.0 for g in gbs]), self.dev)
+        self.gb_kind = to_device(np.array([gbsolv.KERNELS[g.kernel] if g else 0 for g in gbs], dtype=np.int32),
+                                 self.dev)
+        self.gb_alpbet = to_device(np.array([g.alpbet if g else 0.0 for g in gbs]), self.dev)
         self._alpb = any(g is not None and g.alpb for g in gbs)
         self.born = torch.ones((B, na), **f64)
         self.dborn = torch.zeros((B, na), **f64)
@@ -196,7 +196,11 @@ class GeometryState:
         import copy
 
         new = copy.copy(self)
-        kc = keep.cpu().numpy()
+        if torch.is_tensor(keep):
+            kc = keep.cpu().numpy()
+        else:
+            kc = np.asarray(keep, dtype=np.int64)
+            keep = to_device(kc, self.dev)
         for name in self._PER_MOLECULE:
             t = getattr(self, name)
             if t is not None:
@@ -243,7 +247,12 @@ class GeometryState:
         ob, nb = self.bb, other.bb
         if (ob.nat_max, ob.nsh_max, ob.nao_max) != (nb.nat_max, nb.nsh_max, nb.nao_max) or ob.gfn != nb.gfn:
             raise ValueError("scatter needs states of the same parametrization and padded sizes")
-        rows = rows.to(self.dev)
+        if torch.is_tensor(rows):
+            rc = rows.cpu().numpy()
+            rows = rows.to(self.dev)
+        else:
+            rc = np.asarray(rows, dtype=np.int64)
+            rows = to_device(rc, self.dev)
         for name in self._PER_MOLECULE:
             if name == "C":
                 continue
@@ -260,7 +269,6 @@ class GeometryState:
         for k, v in ob.dev.items():
             if k not in self._GLOBAL_DEV:
                 v.index_copy_(0, rows, nb.dev[k])
-        rc = rows.cpu().numpy()
         for k in self._BB_HOST:
             getattr(ob, k)[rc] = getattr(nb, k)
 

@@ -59,6 +59,13 @@ class Request:
         return float(self.finished - self.arrival)
 
 
+def _to_device(a: np.ndarray, dev) -> torch.Tensor:
+    """Host array to the device through pinned memory, without waiting: a
+    copy from pageable memory can wait for the whole device, i.e. for the
+    kernels of the other pools' streams."""
+    return torch.from_numpy(np.ascontiguousarray(a)).pin_memory().to(dev, non_blocking=True)
+
+
 def sizes(numbers: np.ndarray, gfn: str) -> tuple[int, int, int]:
     """(nat, nsh, nao) of a molecule."""
     tab = shell_table(gfn)
@@ -136,6 +143,7 @@ class SlotPool:
         self.req: list[Request | None] = [None] * capacity
         self.pending = None  # launched iteration awaiting finish()
         self.event = torch.cuda.Event()
+        self.retiring: list = []  # (event, requests, host energy, host forces, ...) in flight
 
     @property
     def nfree(self) -> int:
@@ -162,10 +170,11 @@ class SlotPool:
             P[i, : len(r.numbers)] = r.positions
         bb = build_batch(N, self.gfn, self.dev, self.dims)
         chg = np.array([r.charge for r in reqs], dtype=np.float64)
-        st = hscf.GeometryState(bb, torch.as_tensor(P, device=self.dev), charges=chg, opts=self.opts,
+        st = hscf.GeometryState(bb, _to_device(P, self.dev), charges=chg, opts=self.opts,
                                 solvation=[r.solvation for r in reqs])
-        rt = torch.as_tensor(rows, device=self.dev)
-        self.state.scatter(rt, st)
+        rows64 = np.asarray(rows, dtype=np.int64)
+        rt = _to_device(rows64, self.dev)
+        self.state.scatter(rows64, st)
         for t in (self.x, self.y, self.xprev, self.yprev, self.hc, self.hd):
             t.index_fill_(0, rt, 0.0)
         self.k[rows] = 0
@@ -197,11 +206,11 @@ class SlotPool:
         # every host-to-device copy comes before the iteration is enqueued (a
         # copy from pageable memory waits for the stream)
         srows = np.nonzero(self.occupied[:hi] & ~self.fresh[:hi])[0]
-        idx = torch.as_tensor(np.concatenate([self.occupied[:hi], self.occupied[:hi] & ~self.fresh[:hi]]).astype(
-            np.int64), device=self.dev)
+        idx = _to_device(np.concatenate([self.occupied[:hi], self.occupied[:hi] & ~self.fresh[:hi]]).astype(np.int64),
+                         self.dev)
         occ, started = idx[:hi].bool(), idx[hi:].bool()
         if len(srows):
-            rk = torch.as_tensor(np.stack([srows, self.k[srows]]), device=self.dev)
+            rk = _to_device(np.stack([srows, self.k[srows]]), self.dev)
             ri, ki = rk[0], rk[1]
         kmax = int(self.k[:hi][self.occupied[:hi]].max(initial=0))
         x, y = self.x[:hi], self.y[:hi]
@@ -230,14 +239,16 @@ class SlotPool:
         xprev.copy_(torch.where(upd, xnew, xprev))
         yprev.copy_(torch.where(upd, ynew, yprev))
         done = started & (dxn < o.x_tol) & (yn < o.f_tol)
-        return hi, done, srows
+        done_h = torch.empty(done.shape, dtype=torch.bool, pin_memory=True)
+        done_h.copy_(done, non_blocking=True)  # read by finish() once the pool's event has completed
+        return hi, done_h, srows
 
     def finish(self, pending, now_fn) -> list[Request]:
         """Host bookkeeping of a launched iteration; retires finished slots."""
         if pending is None:
             return []
-        hi, done, srows = pending
-        done = done.cpu().numpy()
+        hi, done_h, srows = pending
+        done = done_h.numpy().copy()
         self.iters[srows] += 1
         self.k[srows] += 1
         full = srows[self.k[srows] >= self.hc.shape[1]]
@@ -253,30 +264,54 @@ class SlotPool:
         return self.retire(out, done[out], now_fn)
 
     def retire(self, rows: np.ndarray, conv: np.ndarray, now_fn) -> list[Request]:
+        """Enqueue energy and forces of the finished slots ``rows`` on the
+        pool's stream (on copies of their rows, so the slots are free at once)
+        and copy the results to pinned host memory without waiting; the
+        requests come back from ``collect`` once that work has completed."""
         reqs = [self.req[r] for r in rows]
         na = self.dims[0]
         N = np.zeros((len(rows), na), dtype=np.int64)
         for i, r in enumerate(reqs):
             N[i, : len(r.numbers)] = r.numbers
         bb = build_batch(N, self.gfn, self.dev, self.dims)
-        keep = torch.as_tensor(rows, device=self.dev)
-        sub = self.state.subset(keep, bb=bb)
-        res = hscf.finalize(sub, self.x.index_select(0, keep), self.iters[rows].copy(), conv.copy())
+        rows64 = np.asarray(rows, dtype=np.int64)
+        sub = self.state.subset(rows64, bb=bb)
+        res = hscf.finalize(sub, self.x.index_select(0, _to_device(rows64, self.dev)), self.iters[rows].copy(),
+                            conv.copy())
         f = hscf.forces(res)
-        energy = res.energy.cpu().numpy()
-        frc = f.cpu().numpy()
-        t = now_fn()
-        for i, r in enumerate(reqs):
-            r.energy = float(energy[i])
-            r.forces = frc[i, : len(r.numbers)].copy()
-            r.iterations = int(self.iters[rows[i]])
-            r.converged = bool(conv[i])
-            r.finished = t
+        e_h = torch.empty(res.energy.shape, dtype=torch.float64, pin_memory=True)
+        f_h = torch.empty(f.shape, dtype=torch.float64, pin_memory=True)
+        e_h.copy_(res.energy, non_blocking=True)
+        f_h.copy_(f, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.dev))
+        self.retiring.append((ev, reqs, e_h, f_h, self.iters[rows].copy(), conv.copy(), (res, f)))
         self.occupied[rows] = False
         self.fresh[rows] = False
         for r in rows:
             self.req[r] = None
-        return reqs
+        return self.collect(now_fn)
+
+    def collect(self, now_fn, wait: bool = False) -> list[Request]:
+        """Requests whose energy and forces have reached the host."""
+        out, keep = [], []
+        for item in self.retiring:
+            ev, reqs, e_h, f_h, its, conv, _ = item
+            if not (wait or ev.query()):
+                keep.append(item)
+                continue
+            ev.synchronize()
+            t = now_fn()
+            energy, frc = e_h.numpy(), f_h.numpy()
+            for i, r in enumerate(reqs):
+                r.energy = float(energy[i])
+                r.forces = frc[i, : len(r.numbers)].copy()
+                r.iterations = int(its[i])
+                r.converged = bool(conv[i])
+                r.finished = t
+            out += reqs
+        self.retiring = keep
+        return out
 
 
 @dataclass
@@ -308,7 +343,7 @@ class Engine:
 
     @property
     def busy(self) -> bool:
-        return (any(p.occupied.any() or p.pending is not None for p in self.pools.values())
+        return (any(p.occupied.any() or p.pending is not None or p.retiring for p in self.pools.values())
                 or any(self.queue.values()))
 
     def admit(self) -> None:
@@ -326,6 +361,7 @@ class Engine:
                 continue
             m = min(p.nfree, len(q))
             if m:
+                dflt.wait_stream(p.stream)  # copies of retiring rows are taken before the rows are refilled
                 with torch.cuda.stream(dflt):
                     p.admit(q[:m], now)
                 p.stream.wait_stream(dflt)
@@ -339,6 +375,8 @@ class Engine:
         self.admit()
         done = []
         for p in self.pools.values():
+            if p.retiring:
+                done += p.collect(self.now)
             if p.pending is not None and p.event.query():
                 with torch.cuda.stream(p.stream):
                     done += p.finish(p.pending, self.now)
