@@ -211,8 +211,8 @@ def jacobi_roofline(records) -> dict:
     nbytes = 0.0
     nsweep, nmat = 0, 0
     for e0, e1, n, nao, sw, variant in records:
-        t_ms.append(e0.elapsed_time(e1))
-        sweeps = sw.cpu().numpy()
+        ms, sweeps = e0.elapsed_time(e1), sw.cpu().numpy()
+        t_ms.append(ms)
         nsweep += int(sweeps.sum())
         nmat += len(sweeps)
         for i in range(len(sweeps)):

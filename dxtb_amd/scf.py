@@ -314,6 +314,8 @@ class GeometryState:
             variant = 1 if n <= JACOBI_SCALAR_MAX else 2
             prof = JACOBI_PROFILE
             sw = None
+            if prof is not None and torch.cuda.is_current_stream_capturing():
+                prof = None  # timing events cannot be read back from a captured graph
             if prof is not None:
                 sw = torch.zeros(self.bb.nbatch, dtype=torch.int32, device=self.dev)
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))

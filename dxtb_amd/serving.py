@@ -32,6 +32,8 @@ import torch
 from . import scf as hscf
 from .basis import build_batch, shell_table
 
+# static window sizes of the captured iteration graphs
+GRAPH_WINDOWS = (8, 16, 32, 64, 128, 256)
 # basis-size classes (padded nao) of the slot pools
 POOL_NAO = (48, 96, 144, 192, 256, 320, 384, 448, 512, 640, 768, 1024, 1280, 1536)
 # padded atom and shell counts of a class, as fractions of its nao; a molecule
@@ -108,7 +110,7 @@ def default_capacity(nao_max: int, budget_bytes: float = 8e9) -> int:
 class SlotPool:
     """Fixed-capacity slots of one basis-size class."""
 
-    def __init__(self, gfn: str, nao_max: int, capacity: int, device, opts: hscf.SCFOptions):
+    def __init__(self, gfn: str, nao_max: int, capacity: int, device, opts: hscf.SCFOptions, graphs: bool = False):
         self.gfn = gfn
         self.dims = class_dims(nao_max)
         self.cap = capacity
@@ -141,6 +143,16 @@ class SlotPool:
         self.occupied = np.zeros(capacity, dtype=bool)
         self.fresh = np.zeros(capacity, dtype=bool)  # admitted, first map evaluation pending
         self.req: list[Request | None] = [None] * capacity
+        # per-slot control of the iteration (occupied, started, history index)
+        # and its convergence flags, at fixed addresses for graph replay
+        self.d_ctl = torch.zeros((3, capacity), dtype=torch.int64, device=self.dev)
+        self.d_done = torch.zeros(capacity, dtype=torch.bool, device=self.dev)
+        self.d_ar = torch.arange(capacity, device=self.dev)
+        self._wins: dict = {}
+        # graph mode needs a capturable eigensolver (the Jacobi kernels)
+        self.graphs = graphs and nao_max <= self.state.lib.dxtb_jacobi_max_n()
+        self._graphs: dict = {}
+        self._gpool = torch.cuda.graph_pool_handle() if self.graphs else None
         self.pending = None  # launched iteration awaiting finish()
         self.event = torch.cuda.Event()
         self.retiring: list = []  # (event, requests, host energy, host forces, ...) in flight
@@ -195,27 +207,32 @@ class SlotPool:
         coef = torch.bmm(B[:, :kmax], (w * v).unsqueeze(-1))
         return r + torch.bmm(A[:, :kmax].transpose(1, 2), coef).squeeze(-1)
 
-    def launch(self):
-        """Enqueue one iteration over the occupied window; returns what
-        ``finish`` needs (device tensors, no host synchronisation)."""
-        hi = self.hi
-        if hi == 0:
-            return None
+    # ---- one SCF iteration of the occupied window --------------------------
+    def _window(self, hi: int) -> int:
+        """Rows iterated: the occupied prefix, rounded up to a static size in
+        graph mode (one captured graph per size)."""
+        if not self.graphs:
+            return hi
+        for w in GRAPH_WINDOWS:
+            if hi <= w <= self.cap:
+                return w
+        return self.cap
+
+    def _iterate(self, W: int, kmax: int) -> None:
+        """One iteration over rows [0, W): Broyden step of the started slots,
+        the SCF map of every row, Broyden update at each slot's own history
+        index, convergence flags into ``d_done``.  Reads the per-slot control
+        (occupied, started, history index) from ``d_ctl``; no host
+        synchronisation, fixed shapes (capturable)."""
         o = self.opts
-        win = self.state.window(hi)
-        # every host-to-device copy comes before the iteration is enqueued (a
-        # copy from pageable memory waits for the stream)
-        srows = np.nonzero(self.occupied[:hi] & ~self.fresh[:hi])[0]
-        idx = _to_device(np.concatenate([self.occupied[:hi], self.occupied[:hi] & ~self.fresh[:hi]]).astype(np.int64),
-                         self.dev)
-        occ, started = idx[:hi].bool(), idx[hi:].bool()
-        if len(srows):
-            rk = _to_device(np.stack([srows, self.k[srows]]), self.dev)
-            ri, ki = rk[0], rk[1]
-        kmax = int(self.k[:hi][self.occupied[:hi]].max(initial=0))
-        x, y = self.x[:hi], self.y[:hi]
-        xprev, yprev = self.xprev[:hi], self.yprev[:hi]
-        hc, hd = self.hc[:hi], self.hd[:hi]
+        win = self._wins.get(W)
+        if win is None:
+            win = self._wins[W] = self.state.window(W)
+        occ, started = self.d_ctl[0, :W].bool(), self.d_ctl[1, :W].bool()
+        kslot = self.d_ctl[2, :W].clamp(max=self.hc.shape[1] - 1)
+        x, y = self.x[:W], self.y[:W]
+        xprev, yprev = self.xprev[:W], self.yprev[:W]
+        hc, hd = self.hc[:W], self.hd[:W]
         w = win.wvec
         dx = -self._mv(hc, hd, y, kmax, w)
         dxn = hscf.wnorm(dx, w)
@@ -229,18 +246,49 @@ class SlotPool:
         ok = started.unsqueeze(-1) & (den != 0)
         cnew = torch.where(ok, ddx - self._mv(hc, hd, ddy, kmax, w), 0.0)
         dnew = torch.where(ok, v / torch.where(den == 0, 1.0, den), 0.0)
-        if len(srows):
-            hc[ri, ki] = cnew[ri]
-            hd[ri, ki] = dnew[ri]
+        ar = self.d_ar[:W]
+        keep = ~started.unsqueeze(-1)
+        hc[ar, kslot] = torch.where(keep, hc[ar, kslot], cnew)
+        hd[ar, kslot] = torch.where(keep, hd[ar, kslot], dnew)
         # the new point becomes the current one for every occupied slot
         upd = occ.unsqueeze(-1)
         x.copy_(torch.where(upd, xnew, x))
         y.copy_(torch.where(upd, ynew, y))
         xprev.copy_(torch.where(upd, xnew, xprev))
         yprev.copy_(torch.where(upd, ynew, yprev))
-        done = started & (dxn < o.x_tol) & (yn < o.f_tol)
-        done_h = torch.empty(done.shape, dtype=torch.bool, pin_memory=True)
-        done_h.copy_(done, non_blocking=True)  # read by finish() once the pool's event has completed
+        self.d_done[:W].copy_(started & (dxn < o.x_tol) & (yn < o.f_tol))
+
+    def launch(self):
+        """Enqueue one iteration over the occupied window on the current
+        (pool) stream; returns what ``finish`` needs.  In graph mode the
+        iteration is one replay of a captured graph (captured after an eager
+        iteration of the same window)."""
+        hi = self.hi
+        if hi == 0:
+            return None
+        W = self._window(hi)
+        srows = np.nonzero(self.occupied[:hi] & ~self.fresh[:hi])[0]
+        ctl = np.zeros((3, W), dtype=np.int64)
+        ctl[0, :hi] = self.occupied[:hi]
+        ctl[1, :hi] = self.occupied[:hi] & ~self.fresh[:hi]
+        ctl[2, :hi] = self.k[:hi]
+        # every host-to-device copy comes before the iteration is enqueued
+        self.d_ctl[:, :W].copy_(torch.from_numpy(ctl).pin_memory(), non_blocking=True)
+        if self.graphs:
+            key = (W, self.state.any_gb, self.state._alpb)
+            g = self._graphs.get(key)
+            if g is None:
+                self._iterate(W, self.hc.shape[1])  # eager: warms up, and is this iteration
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, pool=self._gpool, stream=torch.cuda.current_stream(self.dev)):
+                    self._iterate(W, self.hc.shape[1])
+                self._graphs[key] = g
+            else:
+                g.replay()
+        else:
+            self._iterate(W, int(self.k[:hi][self.occupied[:hi]].max(initial=0)))
+        done_h = torch.empty(hi, dtype=torch.bool, pin_memory=True)
+        done_h.copy_(self.d_done[:hi], non_blocking=True)  # read by finish() once the pool's event completed
         return hi, done_h, srows
 
     def finish(self, pending, now_fn) -> list[Request]:
@@ -322,6 +370,10 @@ class Engine:
     device: str | torch.device = "cuda:0"
     opts: hscf.SCFOptions = field(default_factory=hscf.SCFOptions)
     capacity: dict | None = None  # class nao -> slots (default: default_capacity)
+    # one captured hipGraph per pool iteration (pools with nao <= 512): cuts the
+    # host work per iteration, but the static windows and the full Broyden
+    # history cost more device time than that saves on the measured workloads
+    graphs: bool = False
     pools: dict = field(default_factory=dict)
     queue: dict = field(default_factory=dict)  # class -> FIFO of requests
     t0: float = field(default_factory=time.perf_counter)
@@ -332,7 +384,7 @@ class Engine:
     def pool(self, c: int) -> SlotPool:
         if c not in self.pools:
             cap = (self.capacity or {}).get(c, default_capacity(c))
-            p = SlotPool(self.gfn, c, cap, self.device, self.opts)
+            p = SlotPool(self.gfn, c, cap, self.device, self.opts, graphs=self.graphs)
             torch.cuda.synchronize(p.dev)  # pool arrays were filled on the creating stream
             self.pools[c] = p
         return self.pools[c]
