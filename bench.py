@@ -277,7 +277,7 @@ def bench_eigen(args, dev):
     def launch():
         A.copy_(A0)  # the blocked variant works in place
         rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work), 2, 1,
-                                  0.0, _lib.ptr(sw), _lib.stream_ptr(dev))
+                                  0.0, None, _lib.ptr(sw), _lib.stream_ptr(dev))
         _lib.check(rc, "dxtb_jacobi_eigh")
 
     for _ in range(args.warmup):
@@ -290,7 +290,7 @@ def bench_eigen(args, dev):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work), 2, 1,
-                                  0.0, _lib.ptr(sw), _lib.stream_ptr(dev))
+                                  0.0, None, _lib.ptr(sw), _lib.stream_ptr(dev))
         e1.record(stream)
         _lib.check(rc, "dxtb_jacobi_eigh")
         e1.synchronize()

@@ -109,7 +109,7 @@ __device__ __forceinline__ bool stop_sweeps(double off, double off_prev, double 
 __global__ __launch_bounds__(JB) void k_jacobi_scalar(int nmax, const int* __restrict__ nsize,
                                                       const double* __restrict__ Ag, double* __restrict__ VT,
                                                       double* __restrict__ evals, int max_sweeps, double tol,
-                                                      int* __restrict__ sweeps_out) {
+                                                      const double* __restrict__ tol_b, int* __restrict__ sweeps_out) {
   extern __shared__ double lds[];
   constexpr int NP = JMAX_LDS / 2;
   const int b = blockIdx.x;
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(JB) void k_jacobi_scalar(int nmax, const int* __res
     }
     off = block_max(off, red);
     amax = block_max(amax, red);
-    const double thr_conv = tol * fmax(amax, 1.0);
+    const double thr_conv = (tol_b ? tol_b[b] : tol) * fmax(amax, 1.0);
     if (stop_sweeps(off, off_prev, thr_conv, sweep)) break;
     off_prev = off;
     const double thr = fmax(1e-300, 1e-18 * fmax(amax, 1.0));
@@ -394,6 +394,7 @@ __device__ __forceinline__ void block_pair(int k, int s, int nb, int& bi, int& b
 __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const int* __restrict__ nsize, double* __restrict__ Ag,
                                                        double* __restrict__ VT, double* __restrict__ evals,
                                                        double* __restrict__ work, int max_sweeps, double tol,
+                                                       const double* __restrict__ tol_b,
                                                        int* __restrict__ sweeps_out) {
   extern __shared__ double lds_raw[];
   BlockedLds& L = *reinterpret_cast<BlockedLds*>(lds_raw);
@@ -446,7 +447,7 @@ __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const int* __r
     }
     off = block_max<NWB>(off, L.red);  // ends with a barrier: V init and last step visible
     amax = block_max<NWB>(amax, L.red);
-    const double thr = fmax(1e-300, tol * fmax(amax, 1.0));
+    const double thr = fmax(1e-300, (tol_b ? tol_b[b] : tol) * fmax(amax, 1.0));
     if (stop_sweeps(off, off_prev, thr, sweep)) break;
     off_prev = off;
     for (int s = 0; s < nb - 1; ++s) {
@@ -604,7 +605,8 @@ extern "C" size_t dxtb_jacobi_work_size(int nbatch, int nmax) {
 }
 
 extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals,
-                                double* work, int variant, int max_sweeps, double tol, int* sweeps, hipStream_t st) {
+                                double* work, int variant, int max_sweeps, double tol, const double* tol_b,
+                                int* sweeps, hipStream_t st) {
   if (nbatch <= 0 || nmax <= 0 || nmax > JMAX || !nsize || !A || !VT || !evals || max_sweeps < 0 || variant < 0 ||
       variant > 2)
     return DXTB_EINVAL;
@@ -623,9 +625,9 @@ extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* 
   }
   if (scalar)
     hipLaunchKernelGGL(k_jacobi_scalar, dim3((unsigned)nbatch), dim3(JB), scalar_shmem(), st, nmax, nsize, A, VT,
-                       evals, max_sweeps, tol, sweeps);
+                       evals, max_sweeps, tol, tol_b, sweeps);
   else
     hipLaunchKernelGGL(k_jacobi_blocked, dim3((unsigned)nbatch), dim3(JBB), sizeof(BlockedLds), st, nmax, nsize, A,
-                       VT, evals, work, max_sweeps, tol, sweeps);
+                       VT, evals, work, max_sweeps, tol, tol_b, sweeps);
   return (int)hipGetLastError();
 }

@@ -41,6 +41,16 @@ PAD_DIAG = 1.0e3  # Fock diagonal of padding AOs: above every real level
 # errors are then O(1e-12 Eh / gap), far below what the 1e-8 Eh energy and
 # 1e-6 Eh/Bohr force bars can see (the energy is variational in the orbitals).
 JACOBI_TOL = 1.0e-12
+# optional inexact eigensolves while the SCF is far from convergence: the
+# Jacobi bar of a molecule is JACOBI_RES_FACTOR x its current residual |y|,
+# within [JACOBI_TOL, JACOBI_TOL_LOOSE]; at convergence (|y| < 1e-8) the bar is
+# JACOBI_TOL again, and the final evaluation always uses JACOBI_TOL.
+JACOBI_RES_FACTOR = 1.0e-4
+JACOBI_TOL_LOOSE = 1.0e-6
+
+
+def jacobi_tolerance(res: torch.Tensor) -> torch.Tensor:
+    return (JACOBI_RES_FACTOR * res).clamp(JACOBI_TOL, JACOBI_TOL_LOOSE).contiguous()
 # orders up to this use the scalar LDS variant of the Jacobi solver, larger
 # ones the blocked matrix-core variant
 JACOBI_SCALAR_MAX = 64
@@ -59,6 +69,11 @@ class SCFOptions:
     damp: float = 0.5
     alpha: float = -0.5
     eigensolver: str = "jacobi"  # "jacobi": hand-written Jacobi for nao <= 512, else device eigh
+    # Jacobi bar tied to the SCF residual (jacobi_tolerance).  Off: on the
+    # config-2 molecules (small gaps) the inexact early solves cost more
+    # Broyden iterations than they save in sweeps (750 instead of 1011 of 1024
+    # converged within 250 iterations)
+    adaptive_eigensolver: bool = False
 
 
 def _orthogonaliser(spad: torch.Tensor, chunk: int = 32) -> torch.Tensor:
@@ -147,6 +162,7 @@ class GeometryState:
         self.valid_sh = (torch.arange(ns, device=dev)[None, :] < d["nsh"][:, None]).double()
         self.C = None  # previous orbitals (warm start of the Jacobi solver)
         self._jwork = None  # Jacobi workspace (a subset keeps the larger buffer)
+        self._jtol = None  # per-molecule Jacobi tolerance of the next solve (None: JACOBI_TOL)
 
     def _setup_solvation(self, solvation, stream) -> None:
         """Generalized Born solvation per molecule (keps = 0: none): Born
@@ -218,6 +234,7 @@ class GeometryState:
             raise ValueError("subset basis must have the padded sizes of the state")
         new.bb = bb
         new.bs = _lib.basis_struct(bb)
+        new._jtol = None
         return new
 
     def window(self, hi: int) -> "GeometryState":
@@ -238,6 +255,7 @@ class GeometryState:
                             ob.nao_max, host["sh_atom"], host["sh_ush"], host["sh_ao"], host["sh_l"], host["ao_atom"],
                             host["ao_shell"], ob.item_mol[:0], ob.item_pair[:0], np.zeros(10, dtype=np.int32), dev)
         new.bs = _lib.basis_struct(new.bb)
+        new._jtol = None
         return new
 
     def scatter(self, rows: torch.Tensor, other: "GeometryState") -> None:
@@ -322,7 +340,7 @@ class GeometryState:
                 ev[0].record(torch.cuda.current_stream(self.dev))
             rc = self.lib.dxtb_jacobi_eigh(self.bb.nbatch, n, _lib.ptr(self.bb.dev["nao"]), _lib.ptr(A),
                                            _lib.ptr(V), _lib.ptr(e), _lib.ptr(self._jwork), variant, 50, JACOBI_TOL,
-                                           _lib.ptr(sw), _lib.stream_ptr(self.dev))
+                                           _lib.ptr(self._jtol), _lib.ptr(sw), _lib.stream_ptr(self.dev))
             _lib.check(rc, "dxtb_jacobi_eigh")
             if prof is not None:
                 ev[1].record(torch.cuda.current_stream(self.dev))
@@ -487,6 +505,9 @@ def run_scf(state: GeometryState, cull: float = 0.75) -> SCFResult:
     idx = torch.arange(B, device=dev)  # full-batch index of every current row
     cur = state
     x = x_full.clone()
+    adaptive = o.adaptive_eigensolver
+    if adaptive:
+        cur._jtol = torch.full((B,), JACOBI_TOL_LOOSE, dtype=torch.float64, device=dev)
     y = x - cur.g(x)
     mix = BatchedBroyden(o.alpha, o.damp, B, state.nvec, o.maxiter + 1, dev, state.wvec)
     active = torch.ones(B, dtype=torch.bool, device=dev)
@@ -497,6 +518,8 @@ def run_scf(state: GeometryState, cull: float = 0.75) -> SCFResult:
         dx = -mix.mv(y)
         dxn = wnorm(dx, cur.wvec)
         xnew = torch.where(active.unsqueeze(-1), x + o.damp * dx, x)
+        if adaptive:
+            cur._jtol = jacobi_tolerance(wnorm(y, cur.wvec))
         ynew = xnew - cur.g(xnew)
         ynew = torch.where(active.unsqueeze(-1), ynew, y)
         yn = wnorm(ynew, cur.wvec)

@@ -237,6 +237,9 @@ class SlotPool:
         dx = -self._mv(hc, hd, y, kmax, w)
         dxn = hscf.wnorm(dx, w)
         xnew = torch.where(started.unsqueeze(-1), x + o.damp * dx, x)
+        if o.adaptive_eigensolver:  # fresh slots: a loose bar for the cold first solve
+            win._jtol = torch.where(started, hscf.jacobi_tolerance(hscf.wnorm(y, w)),
+                                    hscf.JACOBI_TOL_LOOSE).contiguous()
         ynew = torch.where(occ.unsqueeze(-1), xnew - win.g(xnew), y)
         yn = hscf.wnorm(ynew, w)
         # Broyden update of the started slots at their own history index

@@ -224,10 +224,11 @@ size_t dxtb_jacobi_work_size(int nbatch, int nmax);
  * dxtb_jacobi_work_size bytes (may be NULL for variant 1).  Stops after
  * max_sweeps, when max|offdiag| <= tol * max(1, max|diag|), or when a sweep
  * no longer halves max|offdiag| within 1000x of that bar (rounding floor);
- * sweeps (nbatch) may be NULL.  Replaces the per-iteration eigensolve of
+ * tol_b (nbatch, device, may be NULL) replaces tol per matrix; sweeps
+ * (nbatch) may be NULL.  Replaces the per-iteration eigensolve of
  * scf/pure/ovlp_diag.py:36-62 (xitorch exacteig / torch.linalg.eigh). */
 int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals, double* work,
-                     int variant, int max_sweeps, double tol, int* sweeps, hipStream_t stream);
+                     int variant, int max_sweeps, double tol, const double* tol_b, int* sweeps, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,7 @@ from dxtb_amd import _lib
 pytestmark = pytest.mark.gpu
 
 
-def _jacobi(A, nsize, variant=0, sweeps=60, tol=1e-14):
+def _jacobi(A, nsize, variant=0, sweeps=60, tol=1e-14, tol_b=None):
     lib = _lib.load()
     B, n, _ = A.shape
     A = A.clone().contiguous()  # the blocked variant works in place
@@ -22,7 +22,7 @@ def _jacobi(A, nsize, variant=0, sweeps=60, tol=1e-14):
     ns = torch.as_tensor(nsize, dtype=torch.int32, device=A.device)
     work = torch.empty(max(lib.dxtb_jacobi_work_size(B, n) // 8, 1), dtype=torch.float64, device=A.device)
     rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(V), _lib.ptr(e), _lib.ptr(work), variant,
-                              sweeps, tol, _lib.ptr(sw), _lib.stream_ptr(A.device))
+                              sweeps, tol, _lib.ptr(tol_b), _lib.ptr(sw), _lib.stream_ptr(A.device))
     _lib.check(rc, "dxtb_jacobi_eigh")
     return e, V, sw
 
@@ -75,3 +75,19 @@ def test_degenerate_and_warm(cuda, variant, n):
     d = torch.diag(torch.linspace(-1, 1, n, dtype=torch.float64)) + 1e-6 * (torch.ones(n, n) - torch.eye(n)).double()
     e2, V2, sw = _jacobi(d[None].to(cuda), [n], variant)
     assert int(sw[0]) <= 3
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_per_matrix_tolerance(cuda, variant):
+    """tol_b sets the stopping bar per matrix: a loose bar stops earlier, and
+    the matrix with the tight bar is solved as with the scalar tolerance."""
+    n = 60
+    g = torch.Generator().manual_seed(5)
+    M = torch.randn(2, n, n, generator=g, dtype=torch.float64)
+    A = (0.5 * (M + M.transpose(1, 2))).to(cuda)
+    tb = torch.tensor([1e-4, 1e-14], dtype=torch.float64, device=cuda)
+    e, V, sw = _jacobi(A, [n, n], variant, tol_b=tb)
+    e1, V1, sw1 = _jacobi(A[1:], [n], variant)
+    assert int(sw[0]) < int(sw[1])
+    assert int(sw[1]) == int(sw1[0])
+    torch.testing.assert_close(e[1], e1[0], atol=0, rtol=0)

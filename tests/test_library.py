@@ -52,8 +52,9 @@ def test_jacobi_arguments_without_gpu():
     assert lib.dxtb_jacobi_work_size(0, 33) == 0
     assert lib.dxtb_jacobi_max_n() >= 128
     args = (None,) * 5
-    assert lib.dxtb_jacobi_eigh(0, 10, *args, 0, 10, 1e-14, None, None) == _lib.DXTB_EINVAL
-    assert lib.dxtb_jacobi_eigh(1, lib.dxtb_jacobi_max_n() + 1, *args, 0, 10, 1e-14, None, None) == _lib.DXTB_EINVAL
+    assert lib.dxtb_jacobi_eigh(0, 10, *args, 0, 10, 1e-14, None, None, None) == _lib.DXTB_EINVAL
+    assert lib.dxtb_jacobi_eigh(1, lib.dxtb_jacobi_max_n() + 1, *args, 0, 10, 1e-14, None, None, None) == \
+        _lib.DXTB_EINVAL
 
 
 def test_ctypes_arity_matches_header():
