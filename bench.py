@@ -191,7 +191,7 @@ def jacobi_bytes(n: int, variant: int) -> float:
     """Algorithmic HBM bytes of one sweep: blocked variant, A and VT read and
     written once per outer step; scalar variant (A in LDS), the two VT rows of
     every rotation read and written once per step."""
-    if variant == 2:
+    if variant != 1:
         return jacobi_bytes_per_sweep(n)
     m = n + (n & 1)
     return (m - 1) * 2.0 * n * n * 8
@@ -210,7 +210,7 @@ def jacobi_roofline(records) -> dict:
     flops = 0.0
     nbytes = 0.0
     nsweep, nmat = 0, 0
-    for e0, e1, n, nao, sw, variant in records:
+    for e0, e1, n, nao, sw, variant, _split in records:
         ms, sweeps = e0.elapsed_time(e1), sw.cpu().numpy()
         t_ms.append(ms)
         nsweep += int(sweeps.sum())
@@ -223,7 +223,7 @@ def jacobi_roofline(records) -> dict:
     total_ms = float(np.sum(t_ms))
     achieved = nbytes / (total_ms * 1e-3) / 1e9
     return {
-        "kernel": "k_jacobi_blocked / k_jacobi_scalar (dxtb_jacobi_eigh)",
+        "kernel": "dxtb_jacobi_eigh (k_jacobi_blocked, split form k_bj_*, k_jacobi_scalar)",
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
@@ -276,7 +276,7 @@ def bench_eigen(args, dev):
 
     def launch():
         A.copy_(A0)  # the blocked variant works in place
-        rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work), 2, 1,
+        rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work), 2, 1, 0,
                                   0.0, None, _lib.ptr(sw), _lib.stream_ptr(dev))
         _lib.check(rc, "dxtb_jacobi_eigh")
 
@@ -289,7 +289,7 @@ def bench_eigen(args, dev):
         A.copy_(A0)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work), 2, 1,
+        rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work), 2, 1, 0,
                                   0.0, None, _lib.ptr(sw), _lib.stream_ptr(dev))
         e1.record(stream)
         _lib.check(rc, "dxtb_jacobi_eigh")
@@ -355,7 +355,7 @@ def bench_gfn2_batch(args, dev):
     from dxtb_amd import workloads
 
     nums, poss, chg = workloads.config_batch(2, count=args.nmol)
-    eng = sv.Engine("gfn2", dev)
+    eng = sv.Engine("gfn2", dev, opts=hscf.SCFOptions(jacobi_variant=args.jacobi_variant))
     warm = _stratified(nums, "gfn2", 8)
     for _ in range(args.warmup):
         eng.run(_requests(nums, poss, chg, warm), realtime=False, progress=_progress("warm-up"))
@@ -408,7 +408,7 @@ def bench_serving(args, dev):
     nums, poss, chg = workloads.config_batch(3, count=args.nmol)
     solv = workloads.config3_solvation(len(nums))
     n = len(nums)
-    eng = sv.Engine("gfn2", dev)
+    eng = sv.Engine("gfn2", dev, opts=hscf.SCFOptions(jacobi_variant=args.jacobi_variant))
     for _ in range(args.warmup):
         eng.run(_requests(nums, poss, chg, _stratified(nums, "gfn2", 4), solv), realtime=False,
                 progress=_progress("warm-up"))
@@ -507,6 +507,8 @@ def main():
     ap.add_argument("--nmol", type=int, default=None)
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--jacobi-variant", type=int, default=3, choices=[0, 2, 3],
+                    help="eigensolver form above order 64: 3 split (default), 2 one workgroup per matrix, 0 auto")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))

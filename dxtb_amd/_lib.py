@@ -93,8 +93,8 @@ _SIGNATURES: dict[str, list] = {
     "dxtb_gb_potential": [c_int, c_int] + [c_void_p] * 9,
     "dxtb_gb_forces": [c_int, c_int] + [c_void_p] * 6 + [POINTER(DxtbGbParams)] + [c_void_p] * 5,
     "dxtb_jacobi_work_size": [c_int, c_int],
-    "dxtb_jacobi_eigh": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double,
-                         c_void_p, c_void_p, c_void_p],
+    "dxtb_jacobi_eigh": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                         c_double, c_void_p, c_void_p, c_void_p],
     "dxtb_forces_classical": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 19,
 }
 

@@ -391,208 +391,328 @@ __device__ __forceinline__ void block_pair(int k, int s, int nb, int& bi, int& b
   bj = max(a, c);
 }
 
+// One matrix of the blocked variant.
+struct BjMat {
+  double* A;   // (nmax, nmax), in place
+  double* V;   // eigenvector rows VT
+  double* Qw;  // rotations of the current outer step, npairs x 32 x 32
+  int n, nmax, nlast, npairs, nb;
+};
+
+__device__ __forceinline__ BjMat bj_mat(int b, int nmax, const int* nsize, double* Ag, double* VT, double* work) {
+  BjMat M;
+  const int npad = (nmax + SB - 1) / SB * SB;
+  M.n = min(max(nsize[b], 0), nmax);
+  M.nmax = nmax;
+  M.nlast = max(M.n - 1, 0);
+  M.npairs = (M.n + SB - 1) / SB;  // block pairs (n padded to 32)
+  M.nb = 2 * M.npairs;             // blocks (even)
+  M.A = Ag + (size_t)b * nmax * nmax;
+  M.V = VT + (size_t)b * nmax * nmax;
+  M.Qw = work + (size_t)b * npad * SB;
+  return M;
+}
+
+// Inner phase of block pair k at outer step s, by one wavefront: stage the
+// 32x32 subproblem, sweep it (all pairs at step 0, where every block has a
+// partner; the pairs across the two blocks afterwards), write the diagonal
+// tile and the rotation back.  Returns whether anything was rotated.
+__device__ bool bj_inner_pair(WaveLds& W, const BjMat& M, int k, int s, double thr, int lane) {
+  int bi, bj;
+  block_pair(k, s, M.nb, bi, bj);
+  double v[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int e = lane + 64 * u;
+    const int gr = gidx(e >> 5, bi, bj), gc = gidx(e & 31, bi, bj);
+    const double x = M.A[(size_t)min(gr, M.nlast) * M.nmax + min(gc, M.nlast)];
+    v[u] = x * mask01(gr < M.n && gc < M.n);
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int e = lane + 64 * u;
+    const int r = e >> 5, c = e & 31;
+    W.S[r * SST + c] = v[u];
+    W.Q[r * SST + c] = (r == c) ? 1.0 : 0.0;
+  }
+  wave_sync();
+  const bool any = inner_sweep(W, thr, s > 0, lane);
+  if (any) {
+#pragma unroll 4
+    for (int u = 0; u < 16; ++u) {
+      const int e = lane + 64 * u;
+      const int r = e >> 5, c = e & 31;
+      const int gr = gidx(r, bi, bj), gc = gidx(c, bi, bj);
+      if (gr < M.n && gc < M.n) M.A[(size_t)gr * M.nmax + gc] = W.S[r * SST + c];
+      M.Qw[(size_t)k * SB * SB + e] = W.Q[r * SST + c];
+    }
+  }
+  wave_sync();
+  return any;
+}
+
+// Product item `item` of outer step s, by one wavefront (W.S is its staging
+// tile): an off-diagonal tile (k < k2) becomes Q_k^T A_kk2 Q_k2, a 32-column
+// panel of the VT rows of pair k becomes Q_k^T VT.  rot: flags of the pairs.
+__device__ void bj_product_item(WaveLds& W, const BjMat& M, const int* rot, int item, int s, int lane) {
+  const int npairs = M.npairs, n = M.n, nmax = M.nmax, nlast = M.nlast;
+  const int ntile = npairs * (npairs - 1) / 2;
+  const int npanel = (n + SB - 1) / SB;
+  if (item < ntile) {
+    int k = 0, rem = item;  // tile index -> (k, k2), k < k2
+    while (rem >= npairs - 1 - k) {
+      rem -= npairs - 1 - k;
+      ++k;
+    }
+    const int k2 = k + 1 + rem;
+    const bool rk = rot[k] != 0, rk2 = rot[k2] != 0;
+    if (!rk && !rk2) return;
+    int bi, bj, ci, cj;
+    block_pair(k, s, M.nb, bi, bj);
+    block_pair(k2, s, M.nb, ci, cj);
+    double* A = M.A;
+    auto loadA = [&](int m, int c) -> double {
+      const int gr = gidx(m, bi, bj), gc = gidx(c, ci, cj);
+      const double x = A[(size_t)min(gr, nlast) * nmax + min(gc, nlast)];
+      return x * mask01(gr < n && gc < n);
+    };
+    d4_t acc[2][2];
+    if (rk) {
+      qt_times(M.Qw + (size_t)k * SB * SB, loadA, lane, acc);
+    } else {  // Q_k = 1: T is the tile itself
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) acc[rb][cb][i] = loadA(rb * BS + acc_row(lane, i), cb * BS + (lane & 15));
+    }
+    // T = Q_k^T X -> LDS (row-major, stride SST)
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          W.S[(rb * BS + acc_row(lane, i)) * SST + cb * BS + (lane & 15)] = acc[rb][cb][i];
+    wave_sync();
+    if (rk2) {
+      // U^T = Q_k2^T T^T with U = T Q_k2: the same product, T read transposed
+      auto loadT = [&](int m, int c) -> double { return W.S[c * SST + m]; };
+      qt_times(M.Qw + (size_t)k2 * SB * SB, loadT, lane, acc);
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int uc = rb * BS + acc_row(lane, i), ur = cb * BS + (lane & 15);
+            const int gr = gidx(ur, bi, bj), gc = gidx(uc, ci, cj);
+            if (gr < n && gc < n) {
+              const double v = acc[rb][cb][i];
+              A[(size_t)gr * nmax + gc] = v;
+              A[(size_t)gc * nmax + gr] = v;
+            }
+          }
+    } else {  // Q_k2 = 1: U = T
+      for (int e = lane; e < SB * SB; e += 64) {
+        const int r = e >> 5, c = e & 31;
+        const int gr = gidx(r, bi, bj), gc = gidx(c, ci, cj);
+        if (gr < n && gc < n) {
+          const double v = W.S[r * SST + c];
+          A[(size_t)gr * nmax + gc] = v;
+          A[(size_t)gc * nmax + gr] = v;
+        }
+      }
+    }
+    wave_sync();
+  } else {
+    const int w2 = item - ntile;
+    const int k = w2 / npanel, pc = w2 - k * npanel;
+    if (!rot[k]) return;
+    int bi, bj;
+    block_pair(k, s, M.nb, bi, bj);
+    const int c0 = pc * SB;
+    double* Vb = M.V;
+    auto loadV = [&](int m, int c) -> double {
+      const int gr = gidx(m, bi, bj), gc = c0 + c;
+      const double x = Vb[(size_t)min(gr, nlast) * nmax + min(gc, nlast)];
+      return x * mask01(gr < n && gc < n);
+    };
+    d4_t acc[2][2];
+    qt_times(M.Qw + (size_t)k * SB * SB, loadV, lane, acc);
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int gr = gidx(rb * BS + acc_row(lane, i), bi, bj), gc = c0 + cb * BS + (lane & 15);
+          if (gr < n && gc < n) Vb[(size_t)gr * nmax + gc] = acc[rb][cb][i];
+        }
+  }
+}
+
+__device__ __forceinline__ int bj_nwork(const BjMat& M) {
+  return M.npairs * (M.npairs - 1) / 2 + M.npairs * ((M.n + SB - 1) / SB);
+}
+
+// max |offdiag| and max |diag| of the matrix, by the whole workgroup
+// (eight independent loads in flight per thread); ends with a barrier.
+__device__ void bj_norms(const BjMat& M, double* red, double& off, double& amax) {
+  off = 0.0;
+  amax = 0.0;
+  const int n = M.n, nn = n * n;
+  for (int e0 = threadIdx.x; e0 < nn; e0 += 8 * JBB) {
+    double v[8];
+    bool dg[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = min(e0 + u * JBB, nn - 1);
+      const int i = e / n, j = e - i * n;
+      const double x = M.A[(size_t)i * M.nmax + j];
+      v[u] = fabs(x) * mask01(e0 + u * JBB < nn);
+      dg[u] = i == j;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      off = fmax(off, dg[u] ? 0.0 : v[u]);
+      amax = fmax(amax, dg[u] ? v[u] : 0.0);
+    }
+  }
+  off = block_max<NWB>(off, red);
+  amax = block_max<NWB>(amax, red);
+}
+
+// One workgroup per matrix, all sweeps.  init_v = 0 continues from the VT
+// the split kernels left; sweep_base is then their per-matrix state, whose
+// fourth entry counts the sweeps already done.
 __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const int* __restrict__ nsize, double* __restrict__ Ag,
                                                        double* __restrict__ VT, double* __restrict__ evals,
                                                        double* __restrict__ work, int max_sweeps, double tol,
-                                                       const double* __restrict__ tol_b,
+                                                       const double* __restrict__ tol_b, int init_v,
+                                                       const double* __restrict__ sweep_base,
                                                        int* __restrict__ sweeps_out) {
   extern __shared__ double lds_raw[];
   BlockedLds& L = *reinterpret_cast<BlockedLds*>(lds_raw);
   const int b = blockIdx.x;
-  const int n = min(max(nsize[b], 0), nmax);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int npad = (nmax + SB - 1) / SB * SB;
-  double* A = Ag + (size_t)b * nmax * nmax;
-  double* Vb = VT + (size_t)b * nmax * nmax;
+  const BjMat M = bj_mat(b, nmax, nsize, Ag, VT, work);
   double* eb = evals + (size_t)b * nmax;
-  double* Qw = work + (size_t)b * npad * SB;  // K rotations of 32x32
+  const int base = sweep_base ? (int)sweep_base[4 * b + 3] : 0;
 
+  if (init_v) {
+    for (int e = threadIdx.x; e < nmax * nmax; e += JBB) {
+      const int i = e / nmax, j = e - i * nmax;
+      M.V[e] = (i == j) ? 1.0 : 0.0;
+    }
+  }
+  if (M.n == 0) {
+    for (int i = threadIdx.x; i < nmax; i += JBB) eb[i] = 0.0;
+    if (threadIdx.x == 0 && sweeps_out != nullptr) sweeps_out[b] = base;
+    return;
+  }
+  WaveLds& W = L.w[wv];
+  int sweep = 0;
+  double off_prev = sweep_base ? sweep_base[4 * b] : 0.0;
+  if (sweep_base && sweep_base[4 * b + 2] != 0.0) max_sweeps = 0;  // converged in the split sweeps
+  for (; sweep < max_sweeps; ++sweep) {
+    double off, amax;
+    bj_norms(M, L.red, off, amax);  // ends with a barrier: V init and last step visible
+    const double thr = fmax(1e-300, (tol_b ? tol_b[b] : tol) * fmax(amax, 1.0));
+    if (stop_sweeps(off, off_prev, thr, base + sweep)) break;
+    off_prev = off;
+    for (int s = 0; s < M.nb - 1; ++s) {
+      for (int k = wv; k < M.npairs; k += NWB) {  // inner phase: one wavefront per block pair
+        const bool any = bj_inner_pair(W, M, k, s, thr, lane);
+        if (lane == 0) L.rot[k] = any ? 1 : 0;
+      }
+      __syncthreads();
+      const int nwork = bj_nwork(M);  // product phase: tiles and VT panels
+      for (int item = wv; item < nwork; item += NWB) bj_product_item(W, M, L.rot, item, s, lane);
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < nmax; i += JBB) eb[i] = (i < M.n) ? M.A[(size_t)i * nmax + i] : 0.0;
+  if (threadIdx.x == 0 && sweeps_out != nullptr) sweeps_out[b] = base + sweep;
+}
+
+// ---- split form: each outer step of the blocked variant as two grid-wide
+// launches (one workgroup per (matrix, block pair), then workgroups over the
+// matrix's product items), for a few large matrices that one workgroup each
+// would leave the device idle.  Per-matrix state in the workspace (4 doubles):
+// off of the previous sweep, rotation bar, done flag, sweeps done.
+
+constexpr int SPLIT_MIN_N = 128;     // auto: split form from this order
+constexpr int SPLIT_MAX_BATCH = 64;  // ... for batches below this size
+
+__global__ __launch_bounds__(JBB) void k_bj_init(int nmax, double* __restrict__ VT, double* __restrict__ sst) {
+  const int b = blockIdx.x;
+  double* Vb = VT + (size_t)b * nmax * nmax;
   for (int e = threadIdx.x; e < nmax * nmax; e += JBB) {
     const int i = e / nmax, j = e - i * nmax;
     Vb[e] = (i == j) ? 1.0 : 0.0;
   }
-  if (n == 0) {
-    for (int i = threadIdx.x; i < nmax; i += JBB) eb[i] = 0.0;
-    if (threadIdx.x == 0 && sweeps_out != nullptr) sweeps_out[b] = 0;
-    return;
-  }
-  const int npairs = (n + SB - 1) / SB;  // block pairs (n padded to 32)
-  const int nb = 2 * npairs;             // blocks (even)
-  const int nlast = n - 1;               // clamp for branch-free loads
+  if (threadIdx.x < 4) sst[4 * b + threadIdx.x] = 0.0;
+}
 
-  WaveLds& W = L.w[wv];
-
-  int sweep = 0;
-  double off_prev = 0.0;
-  for (; sweep < max_sweeps; ++sweep) {
-    // max |offdiag| and max |diag|, eight independent loads in flight per thread
-    double off = 0.0, amax = 0.0;
-    const int nn = n * n;
-    for (int e0 = threadIdx.x; e0 < nn; e0 += 8 * JBB) {
-      double v[8];
-      bool dg[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int e = min(e0 + u * JBB, nn - 1);
-        const int i = e / n, j = e - i * n;
-        const double x = A[(size_t)i * nmax + j];
-        v[u] = fabs(x) * mask01(e0 + u * JBB < nn);
-        dg[u] = i == j;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        off = fmax(off, dg[u] ? 0.0 : v[u]);
-        amax = fmax(amax, dg[u] ? v[u] : 0.0);
-      }
-    }
-    off = block_max<NWB>(off, L.red);  // ends with a barrier: V init and last step visible
-    amax = block_max<NWB>(amax, L.red);
-    const double thr = fmax(1e-300, (tol_b ? tol_b[b] : tol) * fmax(amax, 1.0));
-    if (stop_sweeps(off, off_prev, thr, sweep)) break;
-    off_prev = off;
-    for (int s = 0; s < nb - 1; ++s) {
-      // ---- inner phase: one wavefront per block pair
-      for (int k = wv; k < npairs; k += NWB) {
-        int bi, bj;
-        block_pair(k, s, nb, bi, bj);
-        double v[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int e = lane + 64 * u;
-          const int gr = gidx(e >> 5, bi, bj), gc = gidx(e & 31, bi, bj);
-          const double x = A[(size_t)min(gr, nlast) * nmax + min(gc, nlast)];
-          v[u] = x * mask01(gr < n && gc < n);
-        }
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int e = lane + 64 * u;
-          const int r = e >> 5, c = e & 31;
-          W.S[r * SST + c] = v[u];
-          W.Q[r * SST + c] = (r == c) ? 1.0 : 0.0;
-        }
-        wave_sync();
-        // every block meets a partner at step 0: the pairs inside the blocks
-        // are rotated there, later steps rotate only the pairs across blocks
-        const bool any = inner_sweep(W, thr, s > 0, lane);
-        if (lane == 0) L.rot[k] = any ? 1 : 0;
-        if (any) {
-#pragma unroll 4
-          for (int u = 0; u < 16; ++u) {
-            const int e = lane + 64 * u;
-            const int r = e >> 5, c = e & 31;
-            const int gr = gidx(r, bi, bj), gc = gidx(c, bi, bj);
-            if (gr < n && gc < n) A[(size_t)gr * nmax + gc] = W.S[r * SST + c];
-            Qw[(size_t)k * SB * SB + e] = W.Q[r * SST + c];
-          }
-        }
-        wave_sync();
-      }
-      __syncthreads();
-      // ---- product phase: off-diagonal tiles (k < k') and VT panels
-      const int ntile = npairs * (npairs - 1) / 2;
-      const int npanel = (n + SB - 1) / SB;
-      const int nwork = ntile + npairs * npanel;
-      for (int item = wv; item < nwork; item += NWB) {
-        if (item < ntile) {
-          int k = 0, rem = item;  // tile index -> (k, k2), k < k2
-          while (rem >= npairs - 1 - k) {
-            rem -= npairs - 1 - k;
-            ++k;
-          }
-          const int k2 = k + 1 + rem;
-          const bool rk = L.rot[k] != 0, rk2 = L.rot[k2] != 0;
-          if (!rk && !rk2) continue;
-          int bi, bj, ci, cj;
-          block_pair(k, s, nb, bi, bj);
-          block_pair(k2, s, nb, ci, cj);
-          auto loadA = [&](int m, int c) -> double {
-            const int gr = gidx(m, bi, bj), gc = gidx(c, ci, cj);
-            const double x = A[(size_t)min(gr, nlast) * nmax + min(gc, nlast)];
-            return x * mask01(gr < n && gc < n);
-          };
-          d4_t acc[2][2];
-          if (rk) {
-            qt_times(Qw + (size_t)k * SB * SB, loadA, lane, acc);
-          } else {  // Q_k = 1: T is the tile itself
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-              for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) acc[rb][cb][i] = loadA(rb * BS + acc_row(lane, i), cb * BS + (lane & 15));
-          }
-          // T = Q_k^T X -> LDS (row-major, stride SST)
-#pragma unroll
-          for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-              for (int i = 0; i < 4; ++i)
-                W.S[(rb * BS + acc_row(lane, i)) * SST + cb * BS + (lane & 15)] = acc[rb][cb][i];
-          wave_sync();
-          if (rk2) {
-            // U^T = Q_k2^T T^T with U = T Q_k2: the same product, T read transposed
-            auto loadT = [&](int m, int c) -> double { return W.S[c * SST + m]; };
-            qt_times(Qw + (size_t)k2 * SB * SB, loadT, lane, acc);
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-              for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                  const int uc = rb * BS + acc_row(lane, i), ur = cb * BS + (lane & 15);
-                  const int gr = gidx(ur, bi, bj), gc = gidx(uc, ci, cj);
-                  if (gr < n && gc < n) {
-                    const double v = acc[rb][cb][i];
-                    A[(size_t)gr * nmax + gc] = v;
-                    A[(size_t)gc * nmax + gr] = v;
-                  }
-                }
-          } else {  // Q_k2 = 1: U = T
-            for (int e = lane; e < SB * SB; e += 64) {
-              const int r = e >> 5, c = e & 31;
-              const int gr = gidx(r, bi, bj), gc = gidx(c, ci, cj);
-              if (gr < n && gc < n) {
-                const double v = W.S[r * SST + c];
-                A[(size_t)gr * nmax + gc] = v;
-                A[(size_t)gc * nmax + gr] = v;
-              }
-            }
-          }
-          wave_sync();
-        } else {
-          const int w2 = item - ntile;
-          const int k = w2 / npanel, pc = w2 - k * npanel;
-          if (!L.rot[k]) continue;
-          int bi, bj;
-          block_pair(k, s, nb, bi, bj);
-          const int c0 = pc * SB;
-          auto loadV = [&](int m, int c) -> double {
-            const int gr = gidx(m, bi, bj), gc = c0 + c;
-            const double x = Vb[(size_t)min(gr, nlast) * nmax + min(gc, nlast)];
-            return x * mask01(gr < n && gc < n);
-          };
-          d4_t acc[2][2];
-          qt_times(Qw + (size_t)k * SB * SB, loadV, lane, acc);
-#pragma unroll
-          for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                const int gr = gidx(rb * BS + acc_row(lane, i), bi, bj), gc = c0 + cb * BS + (lane & 15);
-                if (gr < n && gc < n) Vb[(size_t)gr * nmax + gc] = acc[rb][cb][i];
-              }
-        }
-      }
-      __syncthreads();
+__global__ __launch_bounds__(JBB) void k_bj_check(int nmax, const int* __restrict__ nsize, double* __restrict__ Ag,
+                                                 double* __restrict__ VT, double* __restrict__ work, double tol,
+                                                 const double* __restrict__ tol_b, double* __restrict__ sst,
+                                                 int sweep) {
+  __shared__ double red[NWB];
+  const int b = blockIdx.x;
+  if (sst[4 * b + 2] != 0.0) return;
+  const BjMat M = bj_mat(b, nmax, nsize, Ag, VT, work);
+  double off, amax;
+  bj_norms(M, red, off, amax);
+  const double thr = fmax(1e-300, (tol_b ? tol_b[b] : tol) * fmax(amax, 1.0));
+  if (threadIdx.x == 0) {
+    if (M.n == 0 || stop_sweeps(off, sst[4 * b + 0], thr, sweep)) {
+      sst[4 * b + 2] = 1.0;
+    } else {
+      sst[4 * b + 0] = off;
+      sst[4 * b + 1] = thr;
+      sst[4 * b + 3] += 1.0;
     }
   }
-  for (int i = threadIdx.x; i < nmax; i += JBB) eb[i] = (i < n) ? A[(size_t)i * nmax + i] : 0.0;
-  if (threadIdx.x == 0 && sweeps_out != nullptr) sweeps_out[b] = sweep;
+}
+
+__global__ __launch_bounds__(64) void k_bj_inner(int nmax, const int* __restrict__ nsize, double* __restrict__ Ag,
+                                                double* __restrict__ VT, double* __restrict__ work,
+                                                const double* __restrict__ sst, int* __restrict__ rot, int s) {
+  __shared__ WaveLds W;
+  const int b = blockIdx.y, k = blockIdx.x;
+  if (sst[4 * b + 2] != 0.0) return;
+  const BjMat M = bj_mat(b, nmax, nsize, Ag, VT, work);
+  if (k >= M.npairs || s >= M.nb - 1) return;
+  const bool any = bj_inner_pair(W, M, k, s, sst[4 * b + 1], threadIdx.x);
+  if (threadIdx.x == 0) rot[b * KMAX + k] = any ? 1 : 0;
+}
+
+__global__ __launch_bounds__(JBB) void k_bj_product(int nmax, const int* __restrict__ nsize, double* __restrict__ Ag,
+                                                   double* __restrict__ VT, double* __restrict__ work,
+                                                   const double* __restrict__ sst, const int* __restrict__ rot,
+                                                   int s) {
+  extern __shared__ double lds_raw[];
+  WaveLds* Ws = reinterpret_cast<WaveLds*>(lds_raw);
+  const int b = blockIdx.y;
+  if (sst[4 * b + 2] != 0.0) return;
+  const BjMat M = bj_mat(b, nmax, nsize, Ag, VT, work);
+  if (s >= M.nb - 1) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int item = blockIdx.x * NWB + wv;
+  if (item < bj_nwork(M)) bj_product_item(Ws[wv], M, rot + b * KMAX, item, s, lane);
 }
 
 bool attr_done[64];
+
+// workspace: rotations (nbatch x npad x 32 doubles), split state
+// (nbatch x 4 doubles), rotation flags (nbatch x KMAX ints)
+size_t rot_bytes(int nbatch, int nmax) {
+  const size_t npad = (size_t)(nmax + SB - 1) / SB * SB;
+  return (size_t)nbatch * npad * SB * sizeof(double);
+}
 
 }  // namespace
 
@@ -600,15 +720,14 @@ extern "C" int dxtb_jacobi_max_n(void) { return JMAX; }
 
 extern "C" size_t dxtb_jacobi_work_size(int nbatch, int nmax) {
   if (nbatch <= 0 || nmax <= 0) return 0;
-  const size_t npad = (size_t)(nmax + SB - 1) / SB * SB;
-  return (size_t)nbatch * npad * SB * sizeof(double);
+  return rot_bytes(nbatch, nmax) + (size_t)nbatch * 4 * sizeof(double) + (size_t)nbatch * KMAX * sizeof(int);
 }
 
 extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals,
-                                double* work, int variant, int max_sweeps, double tol, const double* tol_b,
-                                int* sweeps, hipStream_t st) {
+                                double* work, int variant, int max_sweeps, int split_sweeps, double tol,
+                                const double* tol_b, int* sweeps, hipStream_t st) {
   if (nbatch <= 0 || nmax <= 0 || nmax > JMAX || !nsize || !A || !VT || !evals || max_sweeps < 0 || variant < 0 ||
-      variant > 2)
+      variant > 3 || split_sweeps < 0)
     return DXTB_EINVAL;
   const bool scalar = variant == 1 || (variant == 0 && nmax <= JSCALAR_AUTO);
   if (scalar && nmax > JMAX_LDS) return DXTB_EINVAL;
@@ -621,13 +740,42 @@ extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* 
                                    (int)scalar_shmem()));
     DXTB_CHECK(hipFuncSetAttribute((const void*)k_jacobi_blocked, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)sizeof(BlockedLds)));
+    DXTB_CHECK(hipFuncSetAttribute((const void*)k_bj_product, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(NWB * sizeof(WaveLds))));
     attr_done[dev] = true;
   }
-  if (scalar)
+  if (scalar) {
     hipLaunchKernelGGL(k_jacobi_scalar, dim3((unsigned)nbatch), dim3(JB), scalar_shmem(), st, nmax, nsize, A, VT,
                        evals, max_sweeps, tol, tol_b, sweeps);
-  else
-    hipLaunchKernelGGL(k_jacobi_blocked, dim3((unsigned)nbatch), dim3(JBB), sizeof(BlockedLds), st, nmax, nsize, A,
-                       VT, evals, work, max_sweeps, tol, tol_b, sweeps);
+    return (int)hipGetLastError();
+  }
+  // split form for a few large matrices: one workgroup per matrix would
+  // leave most compute units idle and run the product items of a step one
+  // after another
+  const int nsw = min(max_sweeps, split_sweeps);
+  const bool split = nsw > 0 && (variant == 3 || (variant == 0 && nmax >= SPLIT_MIN_N && nbatch < SPLIT_MAX_BATCH));
+  const double* base = nullptr;
+  if (split) {
+    double* sst = work + rot_bytes(nbatch, nmax) / sizeof(double);
+    int* rot = reinterpret_cast<int*>(sst + 4 * nbatch);
+    const int npairs = (nmax + SB - 1) / SB, nb = 2 * npairs;
+    const int nwork = npairs * (npairs - 1) / 2 + npairs * npairs;
+    hipLaunchKernelGGL(k_bj_init, dim3((unsigned)nbatch), dim3(JBB), 0, st, nmax, VT, sst);
+    for (int sw = 0; sw < nsw; ++sw) {
+      hipLaunchKernelGGL(k_bj_check, dim3((unsigned)nbatch), dim3(JBB), 0, st, nmax, nsize, A, VT, work, tol, tol_b,
+                         sst, sw);
+      for (int s = 0; s < nb - 1; ++s) {
+        hipLaunchKernelGGL(k_bj_inner, dim3((unsigned)npairs, (unsigned)nbatch), dim3(64), 0, st, nmax, nsize, A, VT,
+                           work, sst, rot, s);
+        hipLaunchKernelGGL(k_bj_product, dim3((unsigned)((nwork + NWB - 1) / NWB), (unsigned)nbatch), dim3(JBB),
+                           NWB * sizeof(WaveLds), st, nmax, nsize, A, VT, work, sst, rot, s);
+      }
+    }
+    DXTB_CHECK(hipGetLastError());
+    base = sst;
+    max_sweeps -= nsw;
+  }
+  hipLaunchKernelGGL(k_jacobi_blocked, dim3((unsigned)nbatch), dim3(JBB), sizeof(BlockedLds), st, nmax, nsize, A, VT,
+                     evals, work, max_sweeps, tol, tol_b, split ? 0 : 1, base, sweeps);
   return (int)hipGetLastError();
 }

@@ -74,6 +74,61 @@ class SCFOptions:
     # Broyden iterations than they save in sweeps (750 instead of 1011 of 1024
     # converged within 250 iterations)
     adaptive_eigensolver: bool = False
+    # dxtb_jacobi_eigh form above order 64: 3 the split form (sweeps spread
+    # over many workgroups per matrix), 2 one workgroup per matrix, 0 the
+    # library's choice for a lone batch (split below half the basis size).
+    # The split form's small workgroups share the device with the other slot
+    # pools' kernels: config-2 batch 132.7 molecules/s against 117.6 with 2
+    # and 116.7 with 0
+    jacobi_variant: int = 3
+
+
+class _SweepPredictor:
+    """Sweeps the next Jacobi solve of a state is expected to need.
+
+    dxtb_jacobi_eigh runs that many sweeps in its split form (grid-wide
+    launches over many workgroups per matrix, for a few large matrices) and
+    finishes what is left one workgroup per matrix; a guess above the need
+    costs a few empty launches, one below it whole one-workgroup sweeps.  The
+    guess is the largest count of the last solve whose counts have reached
+    the host (copied back without blocking) plus one, and FRESH where
+    orbitals restart from X (first solve, admitted molecules)."""
+
+    FRESH = 12
+
+    def __init__(self):
+        self.pred = self.FRESH
+        self.bump = False
+        self.sw = self.host = self.ev = None
+        self.n = 0
+
+    def predict(self) -> int:
+        capturing = torch.cuda.is_current_stream_capturing()  # no event queries inside a capture
+        if self.ev is not None and not capturing and self.ev.query():
+            self.pred = int(self.host[: self.n].max()) + 1
+            self.ev = None
+        if self.bump:
+            self.bump = False
+            return max(self.pred, self.FRESH)
+        return self.pred
+
+    def fresh(self) -> None:
+        self.bump = True
+
+    def counts(self, nb: int, dev) -> torch.Tensor:
+        if self.sw is None or self.sw.numel() < nb:
+            self.sw = torch.zeros(nb, dtype=torch.int32, device=dev)
+            self.host = torch.zeros(nb, dtype=torch.int32, pin_memory=True)
+            self.ev = None
+        return self.sw[:nb]
+
+    def record(self, nb: int, dev) -> None:
+        if torch.cuda.is_current_stream_capturing():
+            return
+        self.host[:nb].copy_(self.sw[:nb], non_blocking=True)
+        self.ev = torch.cuda.Event()
+        self.ev.record(torch.cuda.current_stream(dev))
+        self.n = nb
 
 
 def _orthogonaliser(spad: torch.Tensor, chunk: int = 32) -> torch.Tensor:
@@ -163,6 +218,7 @@ class GeometryState:
         self.C = None  # previous orbitals (warm start of the Jacobi solver)
         self._jwork = None  # Jacobi workspace (a subset keeps the larger buffer)
         self._jtol = None  # per-molecule Jacobi tolerance of the next solve (None: JACOBI_TOL)
+        self._sweeps = _SweepPredictor()  # shared by windows and subsets of this state
 
     def _setup_solvation(self, solvation, stream) -> None:
         """Generalized Born solvation per molecule (keps = 0: none): Born
@@ -280,6 +336,7 @@ class GeometryState:
         if self.C is None:
             self.C = self.X.clone()
         self.C.index_copy_(0, rows, other.X)
+        self._sweeps.fresh()
         self.any_gb = self.any_gb or other.any_gb
         self._alpb = self._alpb or other._alpb
         if other.any_gb:
@@ -329,22 +386,25 @@ class GeometryState:
             nbytes = self.lib.dxtb_jacobi_work_size(self.bb.nbatch, n)
             if self._jwork is None or self._jwork.numel() * 8 < nbytes:
                 self._jwork = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=self.dev)
-            variant = 1 if n <= JACOBI_SCALAR_MAX else 2
+            # scalar form up to order 64, else a blocked form (SCFOptions.jacobi_variant)
+            variant = 1 if n <= JACOBI_SCALAR_MAX else self.opts.jacobi_variant
+            nb = self.bb.nbatch
+            split = _SweepPredictor.FRESH if self.C is None else self._sweeps.predict()
+            sw = self._sweeps.counts(nb, self.dev)
             prof = JACOBI_PROFILE
-            sw = None
             if prof is not None and torch.cuda.is_current_stream_capturing():
                 prof = None  # timing events cannot be read back from a captured graph
             if prof is not None:
-                sw = torch.zeros(self.bb.nbatch, dtype=torch.int32, device=self.dev)
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 ev[0].record(torch.cuda.current_stream(self.dev))
-            rc = self.lib.dxtb_jacobi_eigh(self.bb.nbatch, n, _lib.ptr(self.bb.dev["nao"]), _lib.ptr(A),
-                                           _lib.ptr(V), _lib.ptr(e), _lib.ptr(self._jwork), variant, 50, JACOBI_TOL,
+            rc = self.lib.dxtb_jacobi_eigh(nb, n, _lib.ptr(self.bb.dev["nao"]), _lib.ptr(A), _lib.ptr(V),
+                                           _lib.ptr(e), _lib.ptr(self._jwork), variant, 50, split, JACOBI_TOL,
                                            _lib.ptr(self._jtol), _lib.ptr(sw), _lib.stream_ptr(self.dev))
             _lib.check(rc, "dxtb_jacobi_eigh")
+            self._sweeps.record(nb, self.dev)
             if prof is not None:
                 ev[1].record(torch.cuda.current_stream(self.dev))
-                prof.append((ev[0], ev[1], n, np.array(self.bb.nao), sw, variant))
+                prof.append((ev[0], ev[1], n, np.array(self.bb.nao), sw.clone(), variant, split))
             C = torch.bmm(Cb, V.transpose(1, 2).contiguous())  # V holds the eigenvectors as rows
             if self.C is None:
                 self.C = C

@@ -210,13 +210,18 @@ int dxtb_gb_forces(int nbatch, int nat_max, const int* nat, const int* numbers, 
 /* Largest matrix order handled by dxtb_jacobi_eigh. */
 int dxtb_jacobi_max_n(void);
 
-/* Bytes of the workspace dxtb_jacobi_eigh needs for the blocked variant. */
+/* Bytes of the workspace dxtb_jacobi_eigh needs for the blocked variants. */
 size_t dxtb_jacobi_work_size(int nbatch, int nmax);
 
-/* Batched symmetric eigensolver, cyclic two-sided Jacobi, one workgroup per
- * matrix.  variant 1: scalar Jacobi with A in LDS (nmax <= 128); variant 2:
+/* Batched symmetric eigensolver, cyclic two-sided Jacobi.  variant 1: scalar
+ * Jacobi with A in LDS, one workgroup per matrix (nmax <= 128); variant 2:
  * blocked Jacobi (16-row blocks, FP64 matrix-core updates, A in global
- * memory); variant 0 picks by nmax.
+ * memory), one workgroup per matrix; variant 3: the blocked Jacobi whose
+ * first split_sweeps sweeps run as grid-wide launches over many workgroups
+ * per matrix (for a few large matrices; converged matrices skip the rest),
+ * finished by variant 2; variant 0 picks: 1 up to nmax 64, 3 for
+ * nmax >= 128 and nbatch < 64, else 2.  split_sweeps (>= 0) is best set to
+ * the sweeps the caller expects (e.g. of the previous SCF iteration).
  * A (nbatch, nmax, nmax): only the leading nsize[b] block of every matrix is
  * used; the blocked variant overwrites it.  VT (nbatch, nmax, nmax) receives
  * the eigenvectors as ROWS (identity on the padding), evals (nbatch, nmax)
@@ -228,7 +233,8 @@ size_t dxtb_jacobi_work_size(int nbatch, int nmax);
  * (nbatch) may be NULL.  Replaces the per-iteration eigensolve of
  * scf/pure/ovlp_diag.py:36-62 (xitorch exacteig / torch.linalg.eigh). */
 int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals, double* work,
-                     int variant, int max_sweeps, double tol, const double* tol_b, int* sweeps, hipStream_t stream);
+                     int variant, int max_sweeps, int split_sweeps, double tol, const double* tol_b, int* sweeps,
+                     hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,8 @@
 """Hand-written batched Jacobi eigensolver (dxtb_jacobi_eigh) against the
 device eigh: eigenvalues, residual |A v - e v|, orthogonality; padded batches
-with odd and maximal orders and (near-)degenerate spectra."""
+with odd and maximal orders and (near-)degenerate spectra.  Variants: 1 the
+scalar LDS form, 2 the blocked form (one workgroup per matrix), 3 the blocked
+form with its first sweeps split over many workgroups per matrix."""
 
 from __future__ import annotations
 
@@ -12,7 +14,7 @@ from dxtb_amd import _lib
 pytestmark = pytest.mark.gpu
 
 
-def _jacobi(A, nsize, variant=0, sweeps=60, tol=1e-14, tol_b=None):
+def _jacobi(A, nsize, variant=0, sweeps=60, tol=1e-14, tol_b=None, split=4):
     lib = _lib.load()
     B, n, _ = A.shape
     A = A.clone().contiguous()  # the blocked variant works in place
@@ -22,7 +24,7 @@ def _jacobi(A, nsize, variant=0, sweeps=60, tol=1e-14, tol_b=None):
     ns = torch.as_tensor(nsize, dtype=torch.int32, device=A.device)
     work = torch.empty(max(lib.dxtb_jacobi_work_size(B, n) // 8, 1), dtype=torch.float64, device=A.device)
     rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(V), _lib.ptr(e), _lib.ptr(work), variant,
-                              sweeps, tol, _lib.ptr(tol_b), _lib.ptr(sw), _lib.stream_ptr(A.device))
+                              sweeps, split, tol, _lib.ptr(tol_b), _lib.ptr(sw), _lib.stream_ptr(A.device))
     _lib.check(rc, "dxtb_jacobi_eigh")
     return e, V, sw
 
@@ -31,7 +33,7 @@ CASES = [(7, [7, 5, 1, 2]), (33, [33, 32, 17]), (64, [64, 63, 40]), (128, [128, 
          (300, [300, 211, 129]), (512, [512, 333])]
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 @pytest.mark.parametrize("nmax,sizes", CASES)
 def test_random_symmetric(cuda, nmax, sizes, variant):
     if variant == 1 and nmax > 128:
@@ -60,7 +62,7 @@ def test_random_symmetric(cuda, nmax, sizes, variant):
     assert (sw.cpu() < 60).all()
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 @pytest.mark.parametrize("n", [32, 100])
 def test_degenerate_and_warm(cuda, variant, n):
     g = torch.Generator().manual_seed(3)
@@ -77,7 +79,7 @@ def test_degenerate_and_warm(cuda, variant, n):
     assert int(sw[0]) <= 3
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 def test_per_matrix_tolerance(cuda, variant):
     """tol_b sets the stopping bar per matrix: a loose bar stops earlier, and
     the matrix with the tight bar is solved as with the scalar tolerance."""
@@ -91,3 +93,42 @@ def test_per_matrix_tolerance(cuda, variant):
     assert int(sw[0]) < int(sw[1])
     assert int(sw[1]) == int(sw1[0])
     torch.testing.assert_close(e[1], e1[0], atol=0, rtol=0)
+
+
+@pytest.mark.parametrize("nmax,sizes", [(128, [128, 97]), (300, [300, 64, 0, 255]), (512, [512])])
+def test_split_matches_blocked(cuda, nmax, sizes):
+    """The split form runs the same rotations as the one-workgroup form: the
+    same sweep counts and eigenpairs to rounding (an empty matrix included)."""
+    print(f"case jacobi split nmax={nmax} sizes={sizes}", flush=True)
+    g = torch.Generator().manual_seed(11)
+    A = torch.zeros(len(sizes), nmax, nmax, dtype=torch.float64)
+    for b, n in enumerate(sizes):
+        M = torch.randn(n, n, generator=g, dtype=torch.float64)
+        A[b, :n, :n] = 0.5 * (M + M.T)
+    A = A.to(cuda)
+    e2, V2, sw2 = _jacobi(A, sizes, 2)
+    e3, V3, sw3 = _jacobi(A, sizes, 3)
+    assert (sw2.cpu() - sw3.cpu()).abs().max().item() <= 1
+    eps = torch.finfo(torch.float64).eps
+    for b, n in enumerate(sizes):
+        if n == 0:
+            assert int(sw3[b]) == 0 and e3[b].abs().max().item() == 0.0
+            continue
+        scale = max(1.0, e2[b, :n].abs().max().item())
+        torch.testing.assert_close(e3[b, :n], e2[b, :n], atol=4 * n * eps * scale, rtol=0)
+        a = A[b, :n, :n]
+        v = V3[b, :n, :n].T
+        torch.testing.assert_close(a @ v, v * e3[b, :n], atol=16 * n * eps * scale, rtol=0)
+
+
+def test_split_few_sweeps(cuda):
+    """max_sweeps below the split sweeps: the split form stops there too."""
+    n = 200
+    g = torch.Generator().manual_seed(2)
+    M = torch.randn(n, n, generator=g, dtype=torch.float64)
+    A = (0.5 * (M + M.T))[None].to(cuda)
+    for ms in (0, 1):
+        e2, _, sw2 = _jacobi(A, [n], 2, sweeps=ms)
+        e3, _, sw3 = _jacobi(A, [n], 3, sweeps=ms)
+        assert int(sw2[0]) == ms and int(sw3[0]) == ms
+        torch.testing.assert_close(e3, e2, atol=1e-12, rtol=0)

@@ -110,3 +110,29 @@ def test_broyden_trajectory_matches_oracle(cuda, idx):
     assert res.converged[0] and ref["converged"]
     assert abs(int(res.iterations[0]) - int(ref["iterations"])) <= 2
     assert abs(float(res.energy[0]) - float(ref["energy"])) < 1e-8
+
+
+def test_split_jacobi_matches_one_workgroup(cuda):
+    """Two of the largest config-2 molecules (basis above 128 functions):
+    energies and forces of the split eigensolver form agree with the
+    one-workgroup form and with the oracle."""
+    from dxtb_amd import workloads
+
+    nums, poss, _ = workloads.config_batch(2, count=64)
+    order = sorted(range(64), key=lambda i: -len(nums[i]))[:2]
+    z = [nums[i] for i in order]
+    p = [poss[i] for i in order]
+    N, P = workloads.pad(z, p)
+    print(f"case split jacobi: config-2 molecules {order} ({[len(x) for x in z]} atoms)", flush=True)
+    out = []
+    for variant in (3, 2):  # split form, one workgroup per matrix
+        bb = hbasis.build_batch(N, "gfn2", cuda)
+        assert bb.nao_max >= 128
+        st = hscf.GeometryState(bb, torch.as_tensor(P, device=cuda), opts=hscf.SCFOptions(jacobi_variant=variant))
+        res = hscf.run_scf(st)
+        assert bool(res.converged.all())
+        out.append((res.energy.cpu(), hscf.forces(res).cpu()))
+    torch.testing.assert_close(out[0][0], out[1][0], atol=1e-9, rtol=0)
+    torch.testing.assert_close(out[0][1], out[1][1], atol=1e-7, rtol=0)
+    ref = oscf.singlepoint(z[0].tolist(), torch.tensor(p[0]), "gfn2", tol=1e-8, maxiter=300)
+    assert abs(float(out[0][0][0]) - float(ref["energy"])) < ETOL

@@ -47,14 +47,17 @@ def test_invalid_arguments_rejected_without_gpu():
 
 def test_jacobi_arguments_without_gpu():
     lib = _lib.load()
-    # workspace: 32 rows of the order padded to a multiple of 32, per matrix
-    assert lib.dxtb_jacobi_work_size(2, 33) == 2 * 64 * 32 * 8
+    # workspace per matrix: 32 rows of the order padded to a multiple of 32
+    # (rotations), 4 doubles of split-sweep state, 16 int rotation flags
+    assert lib.dxtb_jacobi_work_size(2, 33) == 2 * (64 * 32 * 8 + 4 * 8 + 16 * 4)
     assert lib.dxtb_jacobi_work_size(0, 33) == 0
     assert lib.dxtb_jacobi_max_n() >= 128
     args = (None,) * 5
-    assert lib.dxtb_jacobi_eigh(0, 10, *args, 0, 10, 1e-14, None, None, None) == _lib.DXTB_EINVAL
-    assert lib.dxtb_jacobi_eigh(1, lib.dxtb_jacobi_max_n() + 1, *args, 0, 10, 1e-14, None, None, None) == \
+    assert lib.dxtb_jacobi_eigh(0, 10, *args, 0, 10, 0, 1e-14, None, None, None) == _lib.DXTB_EINVAL
+    assert lib.dxtb_jacobi_eigh(1, lib.dxtb_jacobi_max_n() + 1, *args, 0, 10, 0, 1e-14, None, None, None) == \
         _lib.DXTB_EINVAL
+    assert lib.dxtb_jacobi_eigh(1, 10, *args, 4, 10, 0, 1e-14, None, None, None) == _lib.DXTB_EINVAL
+    assert lib.dxtb_jacobi_eigh(1, 10, *args, 3, 10, -1, 1e-14, None, None, None) == _lib.DXTB_EINVAL
 
 
 def test_ctypes_arity_matches_header():
