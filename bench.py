@@ -274,10 +274,13 @@ def bench_eigen(args, dev):
     dst.copy_(src)
     torch.cuda.synchronize(dev)
 
+    variant = args.eig_variant  # 2: one workgroup per matrix; 3: the split form (its one sweep split)
+    split = 1 if variant == 3 else 0
+
     def launch():
         A.copy_(A0)  # the blocked variant works in place
-        rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work), 2, 1, 0,
-                                  0.0, None, _lib.ptr(sw), _lib.stream_ptr(dev))
+        rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work),
+                                  variant, 1, split, 0.0, None, _lib.ptr(sw), _lib.stream_ptr(dev))
         _lib.check(rc, "dxtb_jacobi_eigh")
 
     for _ in range(args.warmup):
@@ -289,8 +292,8 @@ def bench_eigen(args, dev):
         A.copy_(A0)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work), 2, 1, 0,
-                                  0.0, None, _lib.ptr(sw), _lib.stream_ptr(dev))
+        rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work),
+                                  variant, 1, split, 0.0, None, _lib.ptr(sw), _lib.stream_ptr(dev))
         e1.record(stream)
         _lib.check(rc, "dxtb_jacobi_eigh")
         e1.synchronize()
@@ -301,7 +304,7 @@ def bench_eigen(args, dev):
     nbytes = B * jacobi_bytes_per_sweep(n)
     flops = B * jacobi_flops_per_sweep(n, 2)
     res = {
-        "metric": "batched Jacobi sweeps/s (blocked variant, FP64)",
+        "metric": f"batched Jacobi sweeps/s ({'split' if variant == 3 else 'one-workgroup'} blocked form, FP64)",
         "value": B / t,
         "unit": "matrix-sweeps/s",
         "ms_per_step": wall * 1e3,
@@ -309,7 +312,8 @@ def bench_eigen(args, dev):
         "config": {"workload": f"eigensolver microbenchmark: {B} random symmetric matrices of order {n}, one sweep "
                                "per launch", "kernel_ms": t * 1e3,
                    "fp64_tflops": flops / t / 1e12, "fp64_peak_tflops": FP64_PEAK_TFLOPS},
-        "roofline": {"kernel": "k_jacobi_blocked", "bound": "hbm", "achieved": nbytes / t / 1e9,
+        "roofline": {"kernel": "k_bj_check + k_bj_inner + k_bj_product + k_jacobi_blocked (one dxtb_jacobi_eigh call)"
+                               if variant == 3 else "k_jacobi_blocked", "bound": "hbm", "achieved": nbytes / t / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": nbytes / t / 1e9 / HBM_PEAK_GBS,
                      "traffic": JACOBI_TRAFFIC_RATIO * nbytes, "bytes_per_launch": nbytes},
     }
@@ -504,6 +508,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="gfn2-batch", choices=["gfn2-batch", "serving", "integrals", "eigen"])
     ap.add_argument("--eig-n", type=int, default=240)
+    ap.add_argument("--eig-variant", type=int, default=2, choices=[2, 3])
     ap.add_argument("--nmol", type=int, default=None)
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -304,7 +304,10 @@ __device__ bool inner_sweep(WaveLds& W, double thr, bool cross, int lane) {
     int pp, qq;
     pair32(kk, s, cross, pp, qq);
     double c2, s2;
-    any |= rotation(W.S[pp * SST + pp], W.S[qq * SST + qq], W.S[pp * SST + qq], thr, c2, s2);
+    const bool rot = rotation(W.S[pp * SST + pp], W.S[qq * SST + qq], W.S[pp * SST + qq], thr, c2, s2);
+    // a round of identity rotations changes nothing (common near convergence): skip its updates
+    if (!__any(rot ? 1 : 0)) continue;
+    any = true;
     if (lane < BS) {
       W.rc[lane] = c2;
       W.rs[lane] = s2;
@@ -350,7 +353,7 @@ __device__ bool inner_sweep(WaveLds& W, double thr, bool cross, int lane) {
     }
     wave_sync();
   }
-  return __any(any ? 1 : 0) != 0;
+  return any;
 }
 
 // out(32x32) = Q^T X for the 32x32 operand X given by a loader; Q is the

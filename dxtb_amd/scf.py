@@ -389,7 +389,9 @@ class GeometryState:
             # scalar form up to order 64, else a blocked form (SCFOptions.jacobi_variant)
             variant = 1 if n <= JACOBI_SCALAR_MAX else self.opts.jacobi_variant
             nb = self.bb.nbatch
-            split = _SweepPredictor.FRESH if self.C is None else self._sweeps.predict()
+            # a captured graph replays over many iterations and admissions: FRESH
+            fresh = self.C is None or torch.cuda.is_current_stream_capturing()
+            split = _SweepPredictor.FRESH if fresh else self._sweeps.predict()
             sw = self._sweeps.counts(nb, self.dev)
             prof = JACOBI_PROFILE
             if prof is not None and torch.cuda.is_current_stream_capturing():
