@@ -1,5 +1,5 @@
-// Batched symmetric eigensolvers for the SCF: two-sided cyclic Jacobi, one
-// workgroup per matrix, in two variants.
+// Batched symmetric eigensolvers for the SCF: two-sided cyclic Jacobi in two
+// variants, the blocked one also in a split form (below).
 //
 // Scalar variant (n <= 128, A in LDS).  Every step of a sweep rotates n/2
 // disjoint index pairs chosen by the round-robin (circle) ordering.  The whole
@@ -44,7 +44,7 @@ constexpr int JBB = 512;               // threads per workgroup of the blocked v
 constexpr int NWB = JBB / 64;          // its wavefronts (one block pair or product item each)
 constexpr int JMAX_LDS = 128;          // largest matrix of the scalar variant
 constexpr int JSTRIDE = JMAX_LDS + 1;  // padded LDS row stride (doubles)
-constexpr int JMAX = 512;              // largest matrix overall
+constexpr int JMAX = 1536;             // largest matrix overall
 constexpr int JSCALAR_AUTO = 64;       // auto: scalar variant up to this order
 constexpr int BS = 16;                 // block size of the blocked variant
 constexpr int SB = 2 * BS;             // subproblem order

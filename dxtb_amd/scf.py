@@ -68,7 +68,7 @@ class SCFOptions:
     maxiter: int = 250
     damp: float = 0.5
     alpha: float = -0.5
-    eigensolver: str = "jacobi"  # "jacobi": hand-written Jacobi for nao <= 512, else device eigh
+    eigensolver: str = "jacobi"  # "jacobi": hand-written Jacobi for nao <= 1536, else device eigh
     # Jacobi bar tied to the SCF residual (jacobi_tolerance).  Off: on the
     # config-2 molecules (small gaps) the inexact early solves cost more
     # Broyden iterations than they save in sweeps (750 instead of 1011 of 1024
@@ -83,37 +83,49 @@ class SCFOptions:
     jacobi_variant: int = 3
 
 
-class _SweepPredictor:
-    """Sweeps the next Jacobi solve of a state is expected to need.
+class _SolveHistory:
+    """Per-state history of the Jacobi solves: the sweeps the next solve is
+    expected to need, and when to re-orthogonalise the accumulated rotation.
 
     dxtb_jacobi_eigh runs that many sweeps in its split form (grid-wide
     launches over many workgroups per matrix, for a few large matrices) and
     finishes what is left one workgroup per matrix; a guess above the need
     costs a few empty launches, one below it whole one-workgroup sweeps.  The
     guess is the largest count of the last solve whose counts have reached
-    the host (copied back without blocking) plus one, and FRESH where
-    orbitals restart from X (first solve, admitted molecules)."""
+    the host (copied back without blocking) plus a margin, and FRESH where
+    orbitals restart from X (first solve, admitted molecules).  Above order
+    512 a missing split sweep costs 20-140 ms (one workgroup per matrix) and a
+    spare one about a millisecond, so the margin and FRESH grow by two."""
 
     FRESH = 12
 
-    def __init__(self):
-        self.pred = self.FRESH
+    def __init__(self, n: int = 0):
+        self.margin = 1 if n <= 512 else 3
+        self.fresh_sweeps = self.FRESH + self.margin - 1
+        self.pred = self.fresh_sweeps
         self.bump = False
+        self.calls = 0
         self.sw = self.host = self.ev = None
         self.n = 0
 
     def predict(self) -> int:
         capturing = torch.cuda.is_current_stream_capturing()  # no event queries inside a capture
         if self.ev is not None and not capturing and self.ev.query():
-            self.pred = int(self.host[: self.n].max()) + 1
+            self.pred = int(self.host[: self.n].max()) + self.margin
             self.ev = None
         if self.bump:
             self.bump = False
-            return max(self.pred, self.FRESH)
+            return max(self.pred, self.fresh_sweeps)
         return self.pred
 
     def fresh(self) -> None:
         self.bump = True
+
+    def reorthogonalise(self, every: int = 4) -> bool:
+        """Whether this solve re-orthogonalises the accumulated rotation (every
+        ``every``-th solve; always inside a graph capture, which is replayed)."""
+        self.calls += 1
+        return torch.cuda.is_current_stream_capturing() or self.calls % every == 0
 
     def counts(self, nb: int, dev) -> torch.Tensor:
         if self.sw is None or self.sw.numel() < nb:
@@ -215,10 +227,15 @@ class GeometryState:
                 w[b, ns + 3 * na : ns + 3 * na + 6 * int(bb.nat[b])] = 1.0
         self.wvec = to_device(w, dev)
         self.valid_sh = (torch.arange(ns, device=dev)[None, :] < d["nsh"][:, None]).double()
-        self.C = None  # previous orbitals (warm start of the Jacobi solver)
+        # warm start of the Jacobi solver: the previous orbitals are X U with U
+        # orthogonal (None: identity), U <- U V^T after every solve.  Each V is
+        # orthogonal to ~1e-13 only, and the products drift linearly (3e-11
+        # after 170 iterations of a 628-function molecule: 2e-8 surplus
+        # electrons, 1e-8 Eh), so U is re-orthogonalised every few solves
+        self.U = None
         self._jwork = None  # Jacobi workspace (a subset keeps the larger buffer)
         self._jtol = None  # per-molecule Jacobi tolerance of the next solve (None: JACOBI_TOL)
-        self._sweeps = _SweepPredictor()  # shared by windows and subsets of this state
+        self._jhist = _SolveHistory(bb.nao_max)  # shared by windows and subsets of this state
 
     def _setup_solvation(self, solvation, stream) -> None:
         """Generalized Born solvation per molecule (keps = 0: none): Born
@@ -253,7 +270,7 @@ class GeometryState:
 
     # ---- culling: a state restricted to some molecules of the batch --------
     _PER_MOLECULE = ("pos", "cn", "fsh", "dfsh", "hfsh", "erep", "mrad", "S", "D", "Q", "H0", "X", "nab", "n0",
-                     "valid_sh", "C", "gb_keps", "gb_kind", "gb_alpbet", "gb_adet", "born", "dborn", "wvec")
+                     "valid_sh", "U", "gb_keps", "gb_kind", "gb_alpbet", "gb_adet", "born", "dborn", "wvec")
     _GLOBAL_DEV = ("ush_l", "ush_np", "ush_alpha", "ush_coeff", "item_mol", "item_pair")
 
     _BB_HOST = ("numbers", "nat", "nsh", "nao", "sh_atom", "sh_ush", "sh_ao", "sh_l", "ao_atom", "ao_shell")
@@ -328,15 +345,16 @@ class GeometryState:
             rc = np.asarray(rows, dtype=np.int64)
             rows = to_device(rc, self.dev)
         for name in self._PER_MOLECULE:
-            if name == "C":
+            if name == "U":
                 continue
             t = getattr(self, name)
             if t is not None:
                 t.index_copy_(0, rows, getattr(other, name))
-        if self.C is None:
-            self.C = self.X.clone()
-        self.C.index_copy_(0, rows, other.X)
-        self._sweeps.fresh()
+        n = ob.nao_max
+        if self.U is None:
+            self.U = torch.eye(n, dtype=torch.float64, device=self.dev).expand(ob.nbatch, n, n).contiguous()
+        self.U.index_copy_(0, rows, torch.eye(n, dtype=torch.float64, device=self.dev).expand(len(rc), n, n))
+        self._jhist.fresh()
         self.any_gb = self.any_gb or other.any_gb
         self._alpb = self._alpb or other._alpb
         if other.any_gb:
@@ -372,14 +390,15 @@ class GeometryState:
     def solve(self, F):
         """Orbitals, occupations and density of a Fock matrix.
 
-        nao <= 512: the hand-written Jacobi solver on A = C^T F C with C
-        the previous iteration's orbitals (S-orthonormal, so A is the Fock
-        matrix in the old orbital basis and nearly diagonal; the first
-        iteration uses C = X = L^-T); the new orbitals are C V.  Larger bases
-        (and eigensolver="device"): X^T F X and the device eigh (rocSOLVER)."""
+        nao <= 1536: the hand-written Jacobi solver on A = C^T F C with
+        C = X U the previous iteration's orbitals (S-orthonormal, so A is the
+        Fock matrix in the old orbital basis and nearly diagonal; the first
+        iteration uses U = 1, C = X = L^-T); the new orbitals are C V and U
+        becomes U V.  Larger bases (and eigensolver="device"): X^T F X and
+        the device eigh (rocSOLVER)."""
         n = self.bb.nao_max
         if self.opts.eigensolver == "jacobi" and n <= self.lib.dxtb_jacobi_max_n():
-            Cb = self.X if self.C is None else self.C
+            Cb = self.X if self.U is None else torch.bmm(self.X, self.U)
             A = (Cb.transpose(-1, -2) @ F @ Cb).contiguous()
             V = torch.empty_like(A)
             e = torch.empty((self.bb.nbatch, n), dtype=torch.float64, device=self.dev)
@@ -390,9 +409,9 @@ class GeometryState:
             variant = 1 if n <= JACOBI_SCALAR_MAX else self.opts.jacobi_variant
             nb = self.bb.nbatch
             # a captured graph replays over many iterations and admissions: FRESH
-            fresh = self.C is None or torch.cuda.is_current_stream_capturing()
-            split = _SweepPredictor.FRESH if fresh else self._sweeps.predict()
-            sw = self._sweeps.counts(nb, self.dev)
+            fresh = self.U is None or torch.cuda.is_current_stream_capturing()
+            split = self._jhist.fresh_sweeps if fresh else self._jhist.predict()
+            sw = self._jhist.counts(nb, self.dev)
             prof = JACOBI_PROFILE
             if prof is not None and torch.cuda.is_current_stream_capturing():
                 prof = None  # timing events cannot be read back from a captured graph
@@ -403,15 +422,21 @@ class GeometryState:
                                            _lib.ptr(e), _lib.ptr(self._jwork), variant, 50, split, JACOBI_TOL,
                                            _lib.ptr(self._jtol), _lib.ptr(sw), _lib.stream_ptr(self.dev))
             _lib.check(rc, "dxtb_jacobi_eigh")
-            self._sweeps.record(nb, self.dev)
+            self._jhist.record(nb, self.dev)
             if prof is not None:
                 ev[1].record(torch.cuda.current_stream(self.dev))
                 prof.append((ev[0], ev[1], n, np.array(self.bb.nao), sw.clone(), variant, split))
-            C = torch.bmm(Cb, V.transpose(1, 2).contiguous())  # V holds the eigenvectors as rows
-            if self.C is None:
-                self.C = C
+            Vt = V.transpose(1, 2).contiguous()  # V holds the eigenvectors as rows
+            C = torch.bmm(Cb, Vt)
+            Un = Vt if self.U is None else torch.bmm(self.U, Vt)
+            if self._jhist.reorthogonalise():
+                # one Newton-Schulz step, U (3 - U^T U) / 2: U^T U = 1 + E becomes 1 + O(E^2)
+                G = torch.bmm(Un.transpose(1, 2).contiguous(), Un)
+                Un = 1.5 * Un - 0.5 * torch.bmm(Un, G)
+            if self.U is None:
+                self.U = Un
             else:
-                self.C.copy_(C)  # in place: a window writes through to its parent state
+                self.U.copy_(Un)  # in place: a window writes through to its parent state
         else:
             Fp = self.X.transpose(-1, -2) @ F @ self.X
             side = torch.cuda.current_stream(self.dev) != torch.cuda.default_stream(self.dev)

@@ -126,7 +126,8 @@ class SlotPool:
         pos[:, 1, 2] = 1.4
         bb = build_batch(h2, gfn, self.dev, self.dims)
         self.state = hscf.GeometryState(bb, torch.as_tensor(pos, device=self.dev), opts=opts)
-        self.state.C = self.state.X.clone()
+        self.state.U = torch.eye(nao_max, dtype=torch.float64, device=self.dev).expand(capacity, nao_max,
+                                                                                      nao_max).contiguous()
         nbytes = self.state.lib.dxtb_jacobi_work_size(capacity, nao_max)
         self.state._jwork = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=self.dev)  # shared by windows
         L = self.state.nvec
@@ -373,9 +374,10 @@ class Engine:
     device: str | torch.device = "cuda:0"
     opts: hscf.SCFOptions = field(default_factory=hscf.SCFOptions)
     capacity: dict | None = None  # class nao -> slots (default: default_capacity)
-    # one captured hipGraph per pool iteration (pools with nao <= 512): cuts the
-    # host work per iteration, but the static windows and the full Broyden
-    # history cost more device time than that saves on the measured workloads
+    # one captured hipGraph per pool iteration (pools within the Jacobi
+    # solver's order limit, 1536): cuts the host work per iteration, but the
+    # static windows and the full Broyden history cost more device time than
+    # that saves on the measured workloads
     graphs: bool = False
     pools: dict = field(default_factory=dict)
     queue: dict = field(default_factory=dict)  # class -> FIFO of requests

@@ -30,7 +30,7 @@ def _jacobi(A, nsize, variant=0, sweeps=60, tol=1e-14, tol_b=None, split=4):
 
 
 CASES = [(7, [7, 5, 1, 2]), (33, [33, 32, 17]), (64, [64, 63, 40]), (128, [128, 127, 100, 3]),
-         (300, [300, 211, 129]), (512, [512, 333])]
+         (300, [300, 211, 129]), (512, [512, 333]), (1030, [1030, 700])]
 
 
 @pytest.mark.parametrize("variant", [1, 2, 3])

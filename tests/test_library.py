@@ -48,8 +48,10 @@ def test_invalid_arguments_rejected_without_gpu():
 def test_jacobi_arguments_without_gpu():
     lib = _lib.load()
     # workspace per matrix: 32 rows of the order padded to a multiple of 32
-    # (rotations), 4 doubles of split-sweep state, 16 int rotation flags
-    assert lib.dxtb_jacobi_work_size(2, 33) == 2 * (64 * 32 * 8 + 4 * 8 + 16 * 4)
+    # (rotations), 4 doubles of split-sweep state, 48 int rotation flags
+    # (block pairs of the largest order, 1536)
+    assert lib.dxtb_jacobi_max_n() == 1536
+    assert lib.dxtb_jacobi_work_size(2, 33) == 2 * (64 * 32 * 8 + 4 * 8 + 48 * 4)
     assert lib.dxtb_jacobi_work_size(0, 33) == 0
     assert lib.dxtb_jacobi_max_n() >= 128
     args = (None,) * 5
