@@ -181,10 +181,13 @@ def jacobi_flops_per_sweep(n: int, variant: int) -> float:
     return (nb - 1) * (ntile * 2 * gemm + npairs * npanel * gemm) + npairs * steps_inner * inner
 
 
-# measured HBM traffic of k_jacobi_blocked / its algorithmic bytes: rocprofv3
-# FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the same counters on a
-# copy of known size (eigen workload, 256 x order 240)
-JACOBI_TRAFFIC_RATIO = 1.087
+# measured HBM traffic of one dxtb_jacobi_eigh call / its algorithmic bytes,
+# by form: rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch (summed over the
+# call's kernels), scaled by the same counters on a copy of known size (eigen
+# workload, 256 x order 240, one sweep).  2: one workgroup per matrix; 3 (and
+# 0, which picks 3 or 2): the split form.  The scalar form (1, A in LDS) is
+# not measured and counted at its algorithmic bytes.
+JACOBI_TRAFFIC_RATIO = {0: 1.063, 1: 1.0, 2: 1.073, 3: 1.063}
 
 
 def jacobi_bytes(n: int, variant: int) -> float:
@@ -209,6 +212,7 @@ def jacobi_roofline(records) -> dict:
     t_ms = []
     flops = 0.0
     nbytes = 0.0
+    traffic = 0.0
     nsweep, nmat = 0, 0
     for e0, e1, n, nao, sw, variant, _split in records:
         ms, sweeps = e0.elapsed_time(e1), sw.cpu().numpy()
@@ -220,6 +224,7 @@ def jacobi_roofline(records) -> dict:
             if k:
                 flops += k * jacobi_flops_per_sweep(int(nao[i]), variant)
                 nbytes += k * jacobi_bytes(int(nao[i]), variant)
+                traffic += JACOBI_TRAFFIC_RATIO[variant] * k * jacobi_bytes(int(nao[i]), variant)
     total_ms = float(np.sum(t_ms))
     achieved = nbytes / (total_ms * 1e-3) / 1e9
     return {
@@ -229,7 +234,7 @@ def jacobi_roofline(records) -> dict:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": JACOBI_TRAFFIC_RATIO * nbytes / len(t_ms),
+        "traffic": traffic / len(t_ms),
         "algorithmic_bytes_per_launch": nbytes / len(t_ms),
         "fp64_tflops": flops / (total_ms * 1e-3) / 1e12,
         "fp64_peak_tflops": FP64_PEAK_TFLOPS,
@@ -315,7 +320,7 @@ def bench_eigen(args, dev):
         "roofline": {"kernel": "k_bj_check + k_bj_inner + k_bj_product + k_jacobi_blocked (one dxtb_jacobi_eigh call)"
                                if variant == 3 else "k_jacobi_blocked", "bound": "hbm", "achieved": nbytes / t / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": nbytes / t / 1e9 / HBM_PEAK_GBS,
-                     "traffic": JACOBI_TRAFFIC_RATIO * nbytes, "bytes_per_launch": nbytes},
+                     "traffic": JACOBI_TRAFFIC_RATIO[variant] * nbytes, "bytes_per_launch": nbytes},
     }
     return res, None
 
@@ -359,7 +364,8 @@ def bench_gfn2_batch(args, dev):
     from dxtb_amd import workloads
 
     nums, poss, chg = workloads.config_batch(2, count=args.nmol)
-    eng = sv.Engine("gfn2", dev, opts=hscf.SCFOptions(jacobi_variant=args.jacobi_variant))
+    eng = sv.Engine("gfn2", dev, opts=hscf.SCFOptions(jacobi_variant=args.jacobi_variant),
+                    graphs=args.engine_graphs)
     warm = _stratified(nums, "gfn2", 8)
     for _ in range(args.warmup):
         eng.run(_requests(nums, poss, chg, warm), realtime=False, progress=_progress("warm-up"))
@@ -412,7 +418,8 @@ def bench_serving(args, dev):
     nums, poss, chg = workloads.config_batch(3, count=args.nmol)
     solv = workloads.config3_solvation(len(nums))
     n = len(nums)
-    eng = sv.Engine("gfn2", dev, opts=hscf.SCFOptions(jacobi_variant=args.jacobi_variant))
+    eng = sv.Engine("gfn2", dev, opts=hscf.SCFOptions(jacobi_variant=args.jacobi_variant),
+                    graphs=args.engine_graphs)
     for _ in range(args.warmup):
         eng.run(_requests(nums, poss, chg, _stratified(nums, "gfn2", 4), solv), realtime=False,
                 progress=_progress("warm-up"))
@@ -512,6 +519,8 @@ def main():
     ap.add_argument("--nmol", type=int, default=None)
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--engine-graphs", action="store_true",
+                    help="serving engine iterates each pool by replaying a captured hipGraph")
     ap.add_argument("--jacobi-variant", type=int, default=3, choices=[0, 2, 3],
                     help="eigensolver form above order 64: 3 split (default), 2 one workgroup per matrix, 0 auto")
     args = ap.parse_args()
