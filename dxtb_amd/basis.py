@@ -190,17 +190,19 @@ def build_batch(numbers, gfn: str, device: torch.device | str, dims: tuple[int, 
     sh_atom = np.zeros((B, nsh_max), dtype=np.int32)
     sh_ush = np.zeros((B, nsh_max), dtype=np.int32)
     sh_l = np.zeros((B, nsh_max), dtype=np.int32)
-    for b in range(B):
-        # atoms of molecule b, each repeated by its shell count
-        za = nums[b, : nat[b]]
-        reps = table.count[za]
-        atoms = np.repeat(np.arange(nat[b]), reps)
-        first = np.repeat(table.first[za], reps)
-        within = np.arange(len(atoms)) - np.repeat(np.cumsum(reps) - reps, reps)
-        ush = first + within
-        sh_atom[b, : len(atoms)] = atoms
-        sh_ush[b, : len(atoms)] = ush
-        sh_l[b, : len(atoms)] = table.l[ush]
+    # every atom of the batch repeated by its shell count (padding atoms have
+    # none), vectorised over the batch
+    reps = cnt.ravel()
+    tot = int(reps.sum())
+    if tot:
+        atom = np.repeat(np.tile(np.arange(nat_max), B), reps)
+        mol = np.repeat(np.repeat(np.arange(B), nat_max), reps)
+        starts = np.cumsum(reps) - reps
+        ush = np.repeat(table.first[nums.ravel()], reps) + (np.arange(tot) - np.repeat(starts, reps))
+        pos = np.arange(tot) - np.repeat(np.cumsum(nsh) - nsh, nsh)
+        sh_atom[mol, pos] = atom
+        sh_ush[mol, pos] = ush
+        sh_l[mol, pos] = table.l[ush]
     valid = np.arange(nsh_max)[None, :] < nsh[:, None]
     nsph = np.where(valid, 2 * sh_l + 1, 0)
     sh_ao = (np.cumsum(nsph, 1) - nsph).astype(np.int32)
@@ -212,23 +214,28 @@ def build_batch(numbers, gfn: str, device: torch.device | str, dims: tuple[int, 
         nao_max = int(dims[2])
     ao_atom = np.zeros((B, nao_max), dtype=np.int32)
     ao_shell = np.zeros((B, nao_max), dtype=np.int32)
-    for b in range(B):
-        reps = nsph[b, : nsh[b]]
-        ao_shell[b, : nao[b]] = np.repeat(np.arange(nsh[b]), reps)
-        ao_atom[b, : nao[b]] = sh_atom[b, ao_shell[b, : nao[b]]]
+    reps = nsph.ravel()  # 0 on padding shells
+    tot = int(reps.sum())
+    if tot:
+        shell = np.repeat(np.tile(np.arange(nsh_max), B), reps)
+        mol = np.repeat(np.repeat(np.arange(B), nsh_max), reps)
+        pos = np.arange(tot) - np.repeat(np.cumsum(nao) - nao, nao)
+        ao_shell[mol, pos] = shell
+        ao_atom[mol, pos] = sh_atom[mol, shell]
 
     # ordered shell pairs (A, B) of every molecule, grouped by class 3*lA + lB
     mols, pairs, cls = [], [], []
     for b in range(B):
         n = int(nsh[b])
-        a_idx, b_idx = np.divmod(np.arange(n * n), n)
+        r = np.arange(n, dtype=np.int32)
+        lb = sh_l[b, :n].astype(np.uint8)
         mols.append(np.full(n * n, b, dtype=np.int32))
-        pairs.append(((a_idx << 16) | b_idx).astype(np.int32))
-        cls.append(3 * sh_l[b, a_idx] + sh_l[b, b_idx])
+        pairs.append(((r[:, None] << 16) | r[None, :]).ravel())
+        cls.append((3 * lb[:, None] + lb[None, :]).ravel())
     mols = np.concatenate(mols)
     pairs = np.concatenate(pairs)
     cls = np.concatenate(cls)
-    order = np.argsort(cls, kind="stable")
+    order = np.argsort(cls, kind="stable")  # 8-bit keys: numpy's stable sort is a radix sort
     item_mol = mols[order]
     item_pair = pairs[order]
     counts = np.bincount(cls, minlength=9)
