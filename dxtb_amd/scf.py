@@ -365,6 +365,30 @@ class GeometryState:
         for k in self._BB_HOST:
             getattr(ob, k)[rc] = getattr(nb, k)
 
+    def move(self, src, dst) -> None:
+        """Copy the molecules of rows ``src`` into rows ``dst`` (disjoint), in
+        place: compaction of serving slots, so the iterated window shrinks to
+        the occupied rows.  Views of this state (windows) see the moved rows."""
+        sc = np.asarray(src, dtype=np.int64)
+        dc = np.asarray(dst, dtype=np.int64)
+        if len(sc) != len(dc) or len(np.intersect1d(sc, dc)):
+            raise ValueError("move needs disjoint row sets of equal length")
+        if len(sc) == 0:
+            return
+        st, dt = to_device(sc, self.dev), to_device(dc, self.dev)
+        for name in self._PER_MOLECULE:
+            t = getattr(self, name)
+            if t is not None:
+                t.index_copy_(0, dt, t.index_select(0, st))
+        for k, v in self.bb.dev.items():
+            if k not in self._GLOBAL_DEV:
+                v.index_copy_(0, dt, v.index_select(0, st))
+        for k in self._BB_HOST:
+            a = getattr(self.bb, k)
+            a[dc] = a[sc]
+        if self._jtol is not None:
+            self._jtol.index_copy_(0, dt, self._jtol.index_select(0, st))
+
     # ---- one application of the SCF map -----------------------------------
     def unpack(self, x):
         B, na, ns = self.bb.nbatch, self.bb.nat_max, self.bb.nsh_max

@@ -110,7 +110,8 @@ def default_capacity(nao_max: int, budget_bytes: float = 8e9) -> int:
 class SlotPool:
     """Fixed-capacity slots of one basis-size class."""
 
-    def __init__(self, gfn: str, nao_max: int, capacity: int, device, opts: hscf.SCFOptions, graphs: bool = False):
+    def __init__(self, gfn: str, nao_max: int, capacity: int, device, opts: hscf.SCFOptions, graphs: bool = False,
+                 graph_window_max: int | None = None):
         self.gfn = gfn
         self.dims = class_dims(nao_max)
         self.cap = capacity
@@ -152,11 +153,15 @@ class SlotPool:
         self._wins: dict = {}
         # graph mode needs a capturable eigensolver (the Jacobi kernels)
         self.graphs = graphs and nao_max <= self.state.lib.dxtb_jacobi_max_n()
+        # graphs only for windows up to this many rows (None: every window)
+        self.graph_wmax = capacity if graph_window_max is None else graph_window_max
         self._graphs: dict = {}
         self._gpool = torch.cuda.graph_pool_handle() if self.graphs else None
         self.pending = None  # launched iteration awaiting finish()
         self.event = torch.cuda.Event()
         self.retiring: list = []  # (event, requests, host energy, host forces, ...) in flight
+        self.moves = 0  # slots moved by compaction
+        self.admitted_busy = 0  # requests admitted while other slots were mid-SCF
 
     @property
     def nfree(self) -> int:
@@ -175,6 +180,8 @@ class SlotPool:
         rows = np.nonzero(~self.occupied)[0][:m]
         if len(rows) < m:
             raise RuntimeError("not enough free slots")
+        if self.occupied.any():
+            self.admitted_busy += m
         na = self.dims[0]
         N = np.zeros((m, na), dtype=np.int64)
         P = np.zeros((m, na, 3))
@@ -198,6 +205,31 @@ class SlotPool:
             r.admitted = now
             self.req[row] = r
 
+    def compact(self) -> bool:
+        """Move the occupied slots above the first free ones into those free
+        slots (state, Broyden vectors and history, bookkeeping), so that the
+        window, which runs up to the highest occupied slot, covers only live
+        molecules.  Between iterations, on the pool's stream.  Returns whether
+        anything moved."""
+        occ = np.nonzero(self.occupied)[0]
+        n = len(occ)
+        if n == 0 or occ[-1] < n:
+            return False
+        src = occ[occ >= n]
+        dst = np.nonzero(~self.occupied[:n])[0]
+        self.state.move(src, dst)
+        st, dt = _to_device(src, self.dev), _to_device(dst, self.dev)
+        for t in (self.x, self.y, self.xprev, self.yprev, self.hc, self.hd):
+            t.index_copy_(0, dt, t.index_select(0, st))
+        for a in (self.k, self.iters, self.occupied, self.fresh):
+            a[dst] = a[src]
+        self.occupied[src] = False
+        self.fresh[src] = False
+        for s_, d_ in zip(src, dst):
+            self.req[d_], self.req[s_] = self.req[s_], None
+        self.moves += len(src)
+        return True
+
     # ---- one SCF iteration of every occupied slot --------------------------
     def _mv(self, A, B, v, kmax, w):
         """(g0 + sum_k a_k <b_k, .>_w) v with the weighted inner product of
@@ -209,10 +241,13 @@ class SlotPool:
         return r + torch.bmm(A[:, :kmax].transpose(1, 2), coef).squeeze(-1)
 
     # ---- one SCF iteration of the occupied window --------------------------
+    def _graphed(self, hi: int) -> bool:
+        return self.graphs and hi <= self.graph_wmax
+
     def _window(self, hi: int) -> int:
         """Rows iterated: the occupied prefix, rounded up to a static size in
         graph mode (one captured graph per size)."""
-        if not self.graphs:
+        if not self._graphed(hi):
             return hi
         for w in GRAPH_WINDOWS:
             if hi <= w <= self.cap:
@@ -278,7 +313,7 @@ class SlotPool:
         ctl[2, :hi] = self.k[:hi]
         # every host-to-device copy comes before the iteration is enqueued
         self.d_ctl[:, :W].copy_(torch.from_numpy(ctl).pin_memory(), non_blocking=True)
-        if self.graphs:
+        if self._graphed(hi):
             key = (W, self.state.any_gb, self.state._alpb)
             g = self._graphs.get(key)
             if g is None:
@@ -313,7 +348,12 @@ class SlotPool:
         out = np.nonzero(done | (self.occupied[:hi] & (self.iters[:hi] >= self.opts.maxiter)))[0]
         if len(out) == 0:
             return []
-        return self.retire(out, done[out], now_fn)
+        res = self.retire(out, done[out], now_fn)
+        # a long-running molecule in a high slot would keep the window wide
+        n = int(self.occupied.sum())
+        if self.hi - n >= max(4, n // 4):
+            self.compact()
+        return res
 
     def retire(self, rows: np.ndarray, conv: np.ndarray, now_fn) -> list[Request]:
         """Enqueue energy and forces of the finished slots ``rows`` on the
@@ -379,6 +419,9 @@ class Engine:
     # static windows and the full Broyden history cost more device time than
     # that saves on the measured workloads
     graphs: bool = False
+    # with graphs: only windows of at most this many rows replay a graph (the
+    # latency-bound tail of a pool); None: every window
+    graph_window_max: int | None = None
     pools: dict = field(default_factory=dict)
     queue: dict = field(default_factory=dict)  # class -> FIFO of requests
     t0: float = field(default_factory=time.perf_counter)
@@ -389,7 +432,8 @@ class Engine:
     def pool(self, c: int) -> SlotPool:
         if c not in self.pools:
             cap = (self.capacity or {}).get(c, default_capacity(c))
-            p = SlotPool(self.gfn, c, cap, self.device, self.opts, graphs=self.graphs)
+            p = SlotPool(self.gfn, c, cap, self.device, self.opts, graphs=self.graphs,
+                         graph_window_max=self.graph_window_max)
             torch.cuda.synchronize(p.dev)  # pool arrays were filled on the creating stream
             self.pools[c] = p
         return self.pools[c]
@@ -403,41 +447,48 @@ class Engine:
         return (any(p.occupied.any() or p.pending is not None or p.retiring for p in self.pools.values())
                 or any(self.queue.values()))
 
+    def _admit_into(self, c: int, p: "SlotPool") -> None:
+        """Admit waiting requests of class ``c`` into the free slots of pool
+        ``p`` (no iteration of the pool in flight).  The setup runs on the
+        default stream (it uses the vendor solvers, Cholesky and triangular
+        solve, which are not reliably ordered with a side stream); the pool's
+        stream then waits for it."""
+        q = self.queue.get(c)
+        if not q or p.pending is not None:
+            return
+        m = min(p.nfree, len(q))
+        if m:
+            dflt = torch.cuda.default_stream(torch.device(self.device))
+            dflt.wait_stream(p.stream)  # copies of retiring rows are taken before the rows are refilled
+            with torch.cuda.stream(dflt):
+                p.admit(q[:m], self.now())
+            p.stream.wait_stream(dflt)
+            del q[:m]
+
     def admit(self) -> None:
-        """Admit waiting requests into idle pools (no iteration in flight).
-        The setup runs on the default stream (it uses the vendor solvers,
-        Cholesky and triangular solve, which are not reliably ordered with a
-        side stream); the pool's stream then waits for it."""
-        now = self.now()
-        dflt = torch.cuda.default_stream(torch.device(self.device))
+        """Admit waiting requests into every pool with no iteration in flight
+        (creating the pools of new classes)."""
         for c, q in self.queue.items():
-            if not q:
-                continue
-            p = self.pool(c)
-            if p.pending is not None:
-                continue
-            m = min(p.nfree, len(q))
-            if m:
-                dflt.wait_stream(p.stream)  # copies of retiring rows are taken before the rows are refilled
-                with torch.cuda.stream(dflt):
-                    p.admit(q[:m], now)
-                p.stream.wait_stream(dflt)
-                del q[:m]
+            if q:
+                self._admit_into(c, self.pool(c))
 
     def step(self) -> list[Request]:
-        """One pass of the event loop: admit what fits, finish every pool
-        whose iteration has completed (retiring converged molecules) and
-        launch its next iteration.  Pools advance independently: a pool of
-        small molecules does not wait for a pool of large ones."""
+        """One pass of the event loop: admit into new and idle pools, then
+        for every pool whose iteration has completed: finish it (retiring
+        converged molecules), admit waiting requests into the freed slots and
+        launch the next iteration.  Pools advance independently: a pool of
+        small molecules does not wait for a pool of large ones, and a busy
+        pool takes new molecules between two of its iterations."""
         self.admit()
         done = []
-        for p in self.pools.values():
+        for c, p in self.pools.items():
             if p.retiring:
                 done += p.collect(self.now)
             if p.pending is not None and p.event.query():
                 with torch.cuda.stream(p.stream):
                     done += p.finish(p.pending, self.now)
                 p.pending = None
+                self._admit_into(c, p)
             if p.pending is None and p.occupied.any():
                 with torch.cuda.stream(p.stream):
                     p.pending = p.launch()

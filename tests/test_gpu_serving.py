@@ -101,3 +101,93 @@ def test_graph_replays_match_eager(cuda):
         assert g[rid].converged and e[rid].converged
         assert abs(g[rid].energy - e[rid].energy) < 1e-9
         np.testing.assert_allclose(g[rid].forces, e[rid].forces, atol=1e-7, rtol=0)
+
+
+def test_compaction_and_admission_into_busy_pool(cuda):
+    """One pool class with 8 slots and 14 molecules: molecules are admitted
+    while others are mid-SCF, and when retirements leave holes below a
+    long-running slot the pool compacts (moves slots down).  Every result
+    equals the one-molecule path's."""
+    nums, poss, chg = workloads.config_batch(3, count=400)
+    cls = [sv.pool_class(*sv.sizes(z, "gfn2")) for z in nums]
+    target = 96
+    sel = [i for i in range(len(nums)) if cls[i] == target][:14]
+    assert len(sel) == 14
+    print("case compaction: class", target, "molecules", sel, flush=True)
+    reqs = [sv.Request(i, nums[i], poss[i], float(chg[i])) for i in sel]
+    eng = sv.Engine("gfn2", cuda, capacity={target: 8})
+    out = eng.run([sv.Request(r.rid, r.numbers, r.positions, r.charge) for r in reqs], realtime=False)
+    pool = eng.pools[target]
+    print(f"case compaction: moves {pool.moves}, admitted while busy {pool.admitted_busy}", flush=True)
+    assert len(out) == len(reqs)
+    assert pool.admitted_busy > 0
+    got = {r.rid: r for r in out}
+    for r in reqs:
+        m = got[r.rid]
+        ref = sv.plain_path(sv.Request(r.rid, r.numbers, r.positions, r.charge), "gfn2", cuda)
+        assert m.converged == ref.converged
+        if m.converged:
+            assert abs(m.energy - ref.energy) < 1e-9, (r.rid, m.energy - ref.energy)
+            np.testing.assert_allclose(m.forces, ref.forces, atol=1e-7, rtol=0)
+
+
+def test_compact_moves_slots_exactly(cuda):
+    """Pool compaction: after three iterations of 8 molecules the five lowest
+    slots are dropped; compact() moves the three survivors into slots 0-2
+    bit for bit (potential vectors, Broyden history, geometry state, basis
+    maps), and the moved molecules then converge to the one-molecule path's
+    results."""
+    nums, poss, chg = workloads.config_batch(3, count=400)
+    cls = [sv.pool_class(*sv.sizes(z, "gfn2")) for z in nums]
+    sel = [i for i in range(len(nums)) if cls[i] == 96][:8]
+    reqs = [sv.Request(i, nums[i], poss[i], float(chg[i])) for i in sel]
+    eng = sv.Engine("gfn2", cuda, capacity={96: 8})
+    eng.t0 = 0.0
+    for r in reqs:
+        eng.submit(r)
+    eng.admit()  # setup on the default stream, as the engine does it
+    pool = eng.pools[96]
+    torch.cuda.synchronize()
+    with torch.cuda.stream(pool.stream):
+        for _ in range(3):
+            pend = pool.launch()
+            torch.cuda.synchronize()
+            pool.finish(pend, lambda: 0.0)
+    torch.cuda.synchronize()
+    assert pool.occupied[:8].all(), "no molecule may converge in three iterations"
+    keep = [5, 6, 7]
+    snap = {name: getattr(pool, name)[keep].clone() for name in ("x", "y", "xprev", "yprev", "hc", "hd")}
+    snap.update({name: getattr(pool.state, name)[keep].clone() for name in ("S", "H0", "U", "X", "pos")})
+    host = {k: getattr(pool.state.bb, k)[keep].copy() for k in ("numbers", "nao", "sh_l", "ao_shell")}
+    k_before = pool.k[keep].copy()
+    pool.occupied[:5] = False
+    for r in range(5):
+        pool.req[r] = None
+    with torch.cuda.stream(pool.stream):
+        assert pool.compact()
+    torch.cuda.synchronize()
+    assert pool.occupied.tolist() == [True] * 3 + [False] * 5 and pool.hi == 3 and pool.moves == 3
+    for name in ("x", "y", "xprev", "yprev", "hc", "hd"):
+        assert torch.equal(getattr(pool, name)[:3], snap[name]), name
+    for name in ("S", "H0", "U", "X", "pos"):
+        assert torch.equal(getattr(pool.state, name)[:3], snap[name]), name
+    for k, v in host.items():
+        assert np.array_equal(getattr(pool.state.bb, k)[:3], v), k
+    assert pool.k[:3].tolist() == k_before.tolist()
+    assert [r.rid for r in pool.req[:3]] == [sel[i] for i in keep]
+    out = []
+    with torch.cuda.stream(pool.stream):
+        for _ in range(300):
+            if not pool.occupied.any():
+                break
+            pend = pool.launch()
+            torch.cuda.synchronize()
+            out += pool.finish(pend, lambda: 0.0)
+        out += pool.collect(lambda: 0.0, wait=True)
+    assert sorted(r.rid for r in out) == sorted(sel[i] for i in keep)
+    for m in out:
+        r = next(q for q in reqs if q.rid == m.rid)
+        ref = sv.plain_path(sv.Request(r.rid, r.numbers, r.positions, r.charge), "gfn2", cuda)
+        assert m.converged and ref.converged
+        assert abs(m.energy - ref.energy) < 1e-9, (m.rid, m.energy - ref.energy)
+        np.testing.assert_allclose(m.forces, ref.forces, atol=1e-7, rtol=0)
