@@ -91,18 +91,23 @@ class _SolveHistory:
     launches over many workgroups per matrix, for a few large matrices) and
     finishes what is left one workgroup per matrix; a guess above the need
     costs a few empty launches, one below it whole one-workgroup sweeps.  The
-    guess is the largest count of the last solve whose counts have reached
-    the host (copied back without blocking) plus a margin, and FRESH where
-    orbitals restart from X (first solve, admitted molecules).  Above order
-    512 a missing split sweep costs 20-140 ms (one workgroup per matrix) and a
-    spare one about a millisecond, so the margin and FRESH grow by two."""
+    guess is the largest count over the last HISTORY solves whose counts have
+    reached the host (copied back without blocking) plus a margin, and FRESH
+    where orbitals restart from X (first solve, admitted molecules).  (The
+    slowest matrix of a window varies from solve to solve; with the last
+    solve's count alone 40 % of the large windows fell short, and the
+    config-2 batch ran 10 % slower.)  Above order 512 a missing split sweep
+    costs 20-140 ms (one workgroup per matrix) and a spare one about a
+    millisecond, so the margin and FRESH grow by two."""
 
     FRESH = 12
+    HISTORY = 8
 
     def __init__(self, n: int = 0):
         self.margin = 1 if n <= 512 else 3
         self.fresh_sweeps = self.FRESH + self.margin - 1
         self.pred = self.fresh_sweeps
+        self.recent: list[int] = []
         self.bump = False
         self.calls = 0
         self.sw = self.host = self.ev = None
@@ -111,7 +116,8 @@ class _SolveHistory:
     def predict(self) -> int:
         capturing = torch.cuda.is_current_stream_capturing()  # no event queries inside a capture
         if self.ev is not None and not capturing and self.ev.query():
-            self.pred = int(self.host[: self.n].max()) + self.margin
+            self.recent = (self.recent + [int(self.host[: self.n].max())])[-self.HISTORY:]
+            self.pred = max(self.recent) + self.margin
             self.ev = None
         if self.bump:
             self.bump = False
