@@ -642,12 +642,14 @@ __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const int* __r
 
 // ---- split form: each outer step of the blocked variant as two grid-wide
 // launches (one workgroup per (matrix, block pair), then workgroups over the
-// matrix's product items), for a few large matrices that one workgroup each
-// would leave the device idle.  Per-matrix state in the workspace (4 doubles):
-// off of the previous sweep, rotation bar, done flag, sweeps done.
+// matrix's product items).  For a few large matrices one workgroup each would
+// leave the device idle and run a step's product items one after another; for
+// a full batch it is about as fast as one workgroup per matrix alone, and its
+// small workgroups share the device better with other streams' kernels.
+// Per-matrix state in the workspace (4 doubles): off of the previous sweep,
+// rotation bar, done flag, sweeps done.
 
-constexpr int SPLIT_MIN_N = 128;     // auto: split form from this order
-constexpr int SPLIT_MAX_BATCH = 64;  // ... for batches below this size
+constexpr int SPLIT_MIN_N = 128;  // auto: split form from this order, for batches below half of it
 
 __global__ __launch_bounds__(JBB) void k_bj_init(int nmax, double* __restrict__ VT, double* __restrict__ sst) {
   const int b = blockIdx.x;
@@ -756,7 +758,7 @@ extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* 
   // leave most compute units idle and run the product items of a step one
   // after another
   const int nsw = min(max_sweeps, split_sweeps);
-  const bool split = nsw > 0 && (variant == 3 || (variant == 0 && nmax >= SPLIT_MIN_N && nbatch < SPLIT_MAX_BATCH));
+  const bool split = nsw > 0 && (variant == 3 || (variant == 0 && nmax >= SPLIT_MIN_N && 2 * nbatch < nmax));
   const double* base = nullptr;
   if (split) {
     double* sst = work + rot_bytes(nbatch, nmax) / sizeof(double);
