@@ -166,3 +166,89 @@ def test_jacobi_above_512_matches_device_eigh(cuda):
     # (the two solvers round differently, so the Broyden iteration counts may differ near the bar)
     torch.testing.assert_close(out[0][0], out[1][0], atol=1e-9, rtol=0)
     torch.testing.assert_close(out[0][1], out[1][1], atol=1e-7, rtol=0)
+
+
+def test_forces_central_differences_config2(cuda):
+    """The hand-written forces of 16 seeded config-2 molecules against central
+    finite differences of the GPU energy (h = 1e-4 Bohr) on two atoms each,
+    all displaced geometries in one batch.  The SCF converges the energy to
+    ~1e-13 Eh, so the difference quotient is good to ~1e-8 Eh/Bohr; the bar is
+    the 1e-6 Eh/Bohr force bar."""
+    from dxtb_amd import workloads
+
+    nums, poss, _ = workloads.config_batch(2, count=200)
+    rng = np.random.default_rng(2026)
+    picks = rng.choice(200, size=24, replace=False)
+    h = 1e-4
+    # base geometries first: keep molecules that converge well inside the
+    # iteration limit (small-gap molecules are ill-conditioned: displaced
+    # copies of one that converges in 182 iterations need 210 to over 250)
+    N, P = workloads.pad([nums[i] for i in picks], [poss[i] for i in picks])
+    bb = hbasis.build_batch(N, "gfn2", cuda)
+    res = hscf.run_scf(hscf.GeometryState(bb, torch.as_tensor(P, device=cuda)))
+    ok = [int(i) for i, c, it in zip(picks, res.converged, res.iterations) if c and it <= 120][:16]
+    assert len(ok) == 16
+    F = hscf.forces(res).cpu()
+    items, geoms, zs = [], [], []
+    for i in ok:
+        atoms = rng.choice(len(nums[i]), size=2, replace=False)
+        for a in atoms:
+            for k in range(3):
+                for sgn in (1.0, -1.0):
+                    p = np.array(poss[i], dtype=np.float64)
+                    p[a, k] += sgn * h
+                    geoms.append(p)
+                    zs.append(nums[i])
+                items.append((i, int(a), k))
+    print(f"case config-2 forces vs central differences: {len(ok)} molecules, {len(geoms)} displaced geometries",
+          flush=True)
+    N2, P2 = workloads.pad(zs, geoms)
+    bb2 = hbasis.build_batch(N2, "gfn2", cuda)
+    res2 = hscf.run_scf(hscf.GeometryState(bb2, torch.as_tensor(P2, device=cuda)))
+    assert bool(res2.converged.all())
+    e = res2.energy.cpu().numpy()
+    row = {int(i): r for r, i in enumerate(picks)}
+    worst = 0.0
+    for t, (i, a, k) in enumerate(items):
+        fd = -(e[2 * t] - e[2 * t + 1]) / (2 * h)
+        an = float(F[row[i], a, k])
+        worst = max(worst, abs(fd - an))
+        assert abs(fd - an) < 1e-6, (i, a, k, fd, an)
+    print(f"case config-2 forces vs central differences: max |dF| {worst:.1e}", flush=True)
+
+
+def test_water_cluster_config4(cuda):
+    """Config 4 at full size: 667 rigid waters (2001 atoms, 4002 basis
+    functions, past the Jacobi solver's order limit, so the device eigh
+    runs).  The SCF converges, the forces sum to zero (translation
+    invariance), and two force components match central differences of the
+    energy (h = 1e-4 Bohr)."""
+    from dxtb_amd import workloads
+
+    z, p = workloads.water_cluster()
+    z = np.asarray(z)
+    p = np.asarray(p, dtype=np.float64)
+    print(f"case config-4 water cluster: {len(z)} atoms", flush=True)
+    bb = hbasis.build_batch(z[None], "gfn2", cuda)
+    assert bb.nao_max == 4002
+    res = hscf.run_scf(hscf.GeometryState(bb, torch.as_tensor(p[None], device=cuda)))
+    assert bool(res.converged[0])
+    f = hscf.forces(res)[0].cpu()
+    assert f.sum(0).abs().max().item() < 1e-10
+    h = 1e-4
+    geoms = []
+    comps = [(0, 2), (1001, 0)]
+    for a, k in comps:
+        for sgn in (1.0, -1.0):
+            q = p.copy()
+            q[a, k] += sgn * h
+            geoms.append(q)
+    zz = np.repeat(z[None], len(geoms), 0)
+    bb2 = hbasis.build_batch(zz, "gfn2", cuda)
+    res2 = hscf.run_scf(hscf.GeometryState(bb2, torch.as_tensor(np.stack(geoms), device=cuda)))
+    assert bool(res2.converged.all())
+    e = res2.energy.cpu().numpy()
+    for t, (a, k) in enumerate(comps):
+        fd = -(e[2 * t] - e[2 * t + 1]) / (2 * h)
+        print(f"case config-4: atom {a} axis {k}: analytic {float(f[a, k]):.9f} fd {fd:.9f}", flush=True)
+        assert abs(fd - float(f[a, k])) < 1e-6
