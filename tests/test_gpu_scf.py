@@ -252,3 +252,19 @@ def test_water_cluster_config4(cuda):
         fd = -(e[2 * t] - e[2 * t + 1]) / (2 * h)
         print(f"case config-4: atom {a} axis {k}: analytic {float(f[a, k]):.9f} fd {fd:.9f}", flush=True)
         assert abs(fd - float(f[a, k])) < 1e-6
+
+
+def test_config0_gfn1_vs_oracle(cuda):
+    """Config 0: the seeded 24-atom neutral closed-shell GFN1 molecule, energy
+    and forces against the oracle at the parity bars."""
+    from dxtb_amd import workloads
+
+    nums, poss, _ = workloads.config_batch(0)
+    z, p = np.asarray(nums[0]), np.asarray(poss[0], dtype=np.float64)
+    print(f"case config-0 GFN1 molecule ({len(z)} atoms)", flush=True)
+    res = _run([(z, p)], "gfn1", cuda)
+    assert bool(res.converged[0])
+    f = hscf.forces(res)[0].cpu()
+    ref = oscf.singlepoint(z.tolist(), torch.tensor(p), "gfn1", tol=1e-10, forces=True)
+    assert abs(float(res.energy[0]) - float(ref["energy"])) < ETOL
+    torch.testing.assert_close(f, ref["forces"], atol=1e-6, rtol=0)
