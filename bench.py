@@ -54,6 +54,14 @@ def integral_flops(la, lb, na, nb):
     )
 
 
+# measured HBM traffic of the integral forward (all nine class launches) / its
+# algorithmic output bytes: rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1 (scales
+# from a copy of known size), config-1 workload.  Each thread writes short
+# row segments of its block (8-40 bytes per component), so most writes fill
+# only part of a cache line.
+INTEGRAL_FWD_TRAFFIC_RATIO = 2.57
+
+
 def bench_integrals(args, dev):
     from dxtb_amd import basis as hbasis
     from dxtb_amd import integrals as hint
@@ -129,7 +137,8 @@ def bench_integrals(args, dev):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": INTEGRAL_FWD_TRAFFIC_RATIO * out_bytes,
+            "algorithmic_bytes_per_launch": out_bytes,
             "fwd_ms": fwd,
             "bwd_ms": float(np.mean(bwd_ms)),
             "fp64_tflops_fwd": flops / (fwd * 1e-3) / 1e12,
