@@ -149,6 +149,10 @@ class SlotPool:
         # and its convergence flags, at fixed addresses for graph replay
         self.d_ctl = torch.zeros((3, capacity), dtype=torch.int64, device=self.dev)
         self.d_done = torch.zeros(capacity, dtype=torch.bool, device=self.dev)
+        # pinned staging of both, reused every iteration: an iteration is
+        # launched only after the previous one (and so its copies) completed
+        self.h_ctl = torch.zeros(3 * capacity, dtype=torch.int64, pin_memory=True)
+        self.h_done = torch.zeros(capacity, dtype=torch.bool, pin_memory=True)
         self.d_ar = torch.arange(capacity, device=self.dev)
         self._wins: dict = {}
         # graph mode needs a capturable eigensolver (the Jacobi kernels)
@@ -312,7 +316,9 @@ class SlotPool:
         ctl[1, :hi] = self.occupied[:hi] & ~self.fresh[:hi]
         ctl[2, :hi] = self.k[:hi]
         # every host-to-device copy comes before the iteration is enqueued
-        self.d_ctl[:, :W].copy_(torch.from_numpy(ctl).pin_memory(), non_blocking=True)
+        hv = self.h_ctl[: 3 * W].view(3, W)
+        hv.numpy()[:] = ctl
+        self.d_ctl[:, :W].copy_(hv, non_blocking=True)
         if self._graphed(hi):
             key = (W, self.state.any_gb, self.state._alpb)
             g = self._graphs.get(key)
@@ -326,7 +332,7 @@ class SlotPool:
                 g.replay()
         else:
             self._iterate(W, int(self.k[:hi][self.occupied[:hi]].max(initial=0)))
-        done_h = torch.empty(hi, dtype=torch.bool, pin_memory=True)
+        done_h = self.h_done[:hi]
         done_h.copy_(self.d_done[:hi], non_blocking=True)  # read by finish() once the pool's event completed
         return hi, done_h, srows
 
