@@ -91,14 +91,18 @@ class _SolveHistory:
     launches over many workgroups per matrix, for a few large matrices) and
     finishes what is left one workgroup per matrix; a guess above the need
     costs a few empty launches, one below it whole one-workgroup sweeps.  The
-    guess is the largest count over the last HISTORY solves whose counts have
-    reached the host (copied back without blocking) plus a margin, and FRESH
-    where orbitals restart from X (first solve, admitted molecules).  (The
-    slowest matrix of a window varies from solve to solve; with the last
-    solve's count alone 40 % of the large windows fell short, and the
-    config-2 batch ran 10 % slower.)  Above order 512 a missing split sweep
-    costs 20-140 ms (one workgroup per matrix) and a spare one about a
-    millisecond, so the margin and FRESH grow by two."""
+    guess is the largest count over the last HISTORY solves of windows of
+    the same size class (up to 8, up to 64, more matrices) whose counts have
+    reached the host (copied back without blocking), plus a margin; a class
+    without history yet takes the largest count of any class, and FRESH
+    applies where orbitals restart from X (first solve, admitted molecules).
+    Measured on the config-2 batch: with the last solve's count alone 40 % of
+    the large windows fell short (10 % slower); with one history for all
+    sizes the small windows of a pool's tail inherited the large windows'
+    counts and ran four to six empty sweeps per solve (10 % slower, and twice
+    the run-to-run spread).  Above order 512 a missing split sweep costs
+    20-140 ms (one workgroup per matrix) and a spare one about a millisecond,
+    so the margin and FRESH grow by two."""
 
     FRESH = 12
     HISTORY = 8
@@ -106,23 +110,29 @@ class _SolveHistory:
     def __init__(self, n: int = 0):
         self.margin = 1 if n <= 512 else 3
         self.fresh_sweeps = self.FRESH + self.margin - 1
-        self.pred = self.fresh_sweeps
-        self.recent: list[int] = []
+        self.recent: dict[int, list[int]] = {}  # size class -> last counts
         self.bump = False
         self.calls = 0
         self.sw = self.host = self.ev = None
         self.n = 0
+        self.ev_class = 0
 
-    def predict(self) -> int:
+    @staticmethod
+    def size_class(nb: int) -> int:
+        return 0 if nb <= 8 else (1 if nb <= 64 else 2)
+
+    def predict(self, nb: int) -> int:
         capturing = torch.cuda.is_current_stream_capturing()  # no event queries inside a capture
         if self.ev is not None and not capturing and self.ev.query():
-            self.recent = (self.recent + [int(self.host[: self.n].max())])[-self.HISTORY:]
-            self.pred = max(self.recent) + self.margin
+            h = self.recent.get(self.ev_class, [])
+            self.recent[self.ev_class] = (h + [int(self.host[: self.n].max())])[-self.HISTORY:]
             self.ev = None
+        hist = self.recent.get(self.size_class(nb)) or [v for h in self.recent.values() for v in h]
+        pred = max(hist) + self.margin if hist else self.fresh_sweeps
         if self.bump:
             self.bump = False
-            return max(self.pred, self.fresh_sweeps)
-        return self.pred
+            return max(pred, self.fresh_sweeps)
+        return pred
 
     def fresh(self) -> None:
         self.bump = True
@@ -147,6 +157,7 @@ class _SolveHistory:
         self.ev = torch.cuda.Event()
         self.ev.record(torch.cuda.current_stream(dev))
         self.n = nb
+        self.ev_class = self.size_class(nb)
 
 
 def _orthogonaliser(spad: torch.Tensor, chunk: int = 32) -> torch.Tensor:
@@ -440,7 +451,7 @@ class GeometryState:
             nb = self.bb.nbatch
             # a captured graph replays over many iterations and admissions: FRESH
             fresh = self.U is None or torch.cuda.is_current_stream_capturing()
-            split = self._jhist.fresh_sweeps if fresh else self._jhist.predict()
+            split = self._jhist.fresh_sweeps if fresh else self._jhist.predict(nb)
             sw = self._jhist.counts(nb, self.dev)
             prof = JACOBI_PROFILE
             if prof is not None and torch.cuda.is_current_stream_capturing():

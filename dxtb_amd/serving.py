@@ -484,10 +484,15 @@ class Engine:
         converged molecules), admit waiting requests into the freed slots and
         launch the next iteration.  Pools advance independently: a pool of
         small molecules does not wait for a pool of large ones, and a busy
-        pool takes new molecules between two of its iterations."""
+        pool takes new molecules between two of its iterations.  Pools of
+        larger molecules are served first: their iterations are the longest,
+        so they carry a batch's critical path (in insertion order the
+        config-2 batch ran at 145-172 molecules/s depending on how the pools
+        happened to interleave, largest first at 170-171)."""
         self.admit()
         done = []
-        for c, p in self.pools.items():
+        for c in sorted(self.pools, reverse=True):
+            p = self.pools[c]
             if p.retiring:
                 done += p.collect(self.now)
             if p.pending is not None and p.event.query():

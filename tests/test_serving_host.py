@@ -41,3 +41,29 @@ def test_capacity_bounds():
         cap = sv.default_capacity(c)
         assert 8 <= cap <= 256
     assert sv.default_capacity(sv.POOL_NAO[-1]) < sv.default_capacity(sv.POOL_NAO[0])
+
+
+def test_sweep_prediction_by_window_size(monkeypatch):
+    """The split-sweep guess of the SCF's eigensolves: per window-size class,
+    the largest recent count plus the margin; a class without history takes
+    the largest count of any class; FRESH until anything is known, and after
+    an admission; wider margins above order 512."""
+    import torch
+
+    from dxtb_amd.scf import _SolveHistory
+
+    monkeypatch.setattr(torch.cuda, "is_current_stream_capturing", lambda: False)
+    h = _SolveHistory(240)
+    assert h.predict(4) == h.fresh_sweeps == 12
+    h.recent = {2: [3, 7, 5]}
+    assert h.predict(200) == 8          # large windows: max 7 + 1
+    assert h.predict(5) == 8            # no small-window history yet: any class
+    h.recent[0] = [2, 3]
+    assert h.predict(5) == 4            # small windows: their own history
+    assert h.predict(40) == 8           # medium class empty: largest of any class
+    h.fresh()
+    assert h.predict(5) == 12 and h.predict(5) == 4   # one fresh solve after an admission
+    big = _SolveHistory(700)
+    big.recent = {0: [3]}
+    assert big.predict(2) == 6 and big.fresh_sweeps == 14
+    assert [_SolveHistory.size_class(n) for n in (1, 8, 9, 64, 65)] == [0, 0, 1, 1, 2]
