@@ -237,7 +237,6 @@ size_t scalar_shmem() {
 struct WaveLds {
   double S[SB * SST];  // subproblem; reused as the staging tile of the products
   double Q[SB * SST];  // accumulated rotation of the subproblem
-  double rc[BS], rs[BS];
 };
 
 constexpr int KMAX = JMAX / SB;  // max block pairs
@@ -294,11 +293,30 @@ __device__ __forceinline__ int acc_row(int lane, int i) { return (lane >> 4) + 4
 // pair, identical results), reads the other rotations it needs from LDS, then
 // loads all of its S and Q elements before it writes any of them (the 2x2
 // tiles and the Q row pairs of one step are disjoint).
+// Value of lane L of this lane's row of 16 lanes (DPP row_share, two 32-bit
+// moves per double).  Every lane of the wavefront must be active.
+template <int L>
+__device__ __forceinline__ double row_share(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x150 + L, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x150 + L, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// One sweep of the 32x32 subproblem in W by one wavefront.  Lane j of row r
+// (lane = 16 r + j) computes the rotation of pair kk = (j + r) mod 16 and owns
+// the 2x2 tiles of S in column pair kk and row pairs r + 4u (u < 4); it also
+// owns row (lane mod 32) of Q in the columns of pairs r + 4u and r + 1 + 4u.
+// With this layout the rotation of pair r + 4u sits in lane 4u of the row and
+// that of r + 1 + 4u in lane 4u + 1, for every row, so each lane fetches its
+// coefficients with row_share moves instead of an LDS round trip.  Each S and
+// Q element has one owner per round (disjoint pairs), so a round needs one
+// wavefront barrier, before the next round reads the new diagonal.
 __device__ bool inner_sweep(WaveLds& W, double thr, bool cross, int lane) {
   bool any = false;
-  const int kk = lane & 15;                    // column pair of this lane's S tiles
-  const int k0 = lane >> 4;                    // row pairs k0 + 4u, u < 4
-  const int kq0 = lane >> 5, iq = lane & 31;   // Q items: pairs kq0 + 2u (u < 8), row iq
+  const int r = lane >> 4;
+  const int kk = ((lane & 15) + r) & 15;  // pair of this lane's rotation; its S column pair
+  const int iq = lane & 31;               // Q row
   const int nsteps = cross ? BS : SB - 1;
   for (int s = 0; s < nsteps; ++s) {
     int pp, qq;
@@ -308,48 +326,60 @@ __device__ bool inner_sweep(WaveLds& W, double thr, bool cross, int lane) {
     // a round of identity rotations changes nothing (common near convergence): skip its updates
     if (!__any(rot ? 1 : 0)) continue;
     any = true;
-    if (lane < BS) {
-      W.rc[lane] = c2;
-      W.rs[lane] = s2;
-    }
-    wave_sync();
-    double c1[4], s1[4], x[4][4];
+    double ca[4], sa[4], cb[4], sb[4];  // pairs r + 4u and r + 1 + 4u
+    ca[0] = row_share<0>(c2);
+    sa[0] = row_share<0>(s2);
+    ca[1] = row_share<4>(c2);
+    sa[1] = row_share<4>(s2);
+    ca[2] = row_share<8>(c2);
+    sa[2] = row_share<8>(s2);
+    ca[3] = row_share<12>(c2);
+    sa[3] = row_share<12>(s2);
+    cb[0] = row_share<1>(c2);
+    sb[0] = row_share<1>(s2);
+    cb[1] = row_share<5>(c2);
+    sb[1] = row_share<5>(s2);
+    cb[2] = row_share<9>(c2);
+    sb[2] = row_share<9>(s2);
+    cb[3] = row_share<13>(c2);
+    sb[3] = row_share<13>(s2);
+    double x[4][4];
     int p1[4], q1[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int k = k0 + 4 * u;
-      c1[u] = W.rc[k];
-      s1[u] = W.rs[k];
-      pair32(k, s, cross, p1[u], q1[u]);
+      pair32(r + 4 * u, s, cross, p1[u], q1[u]);
       x[u][0] = W.S[p1[u] * SST + pp];
       x[u][1] = W.S[p1[u] * SST + qq];
       x[u][2] = W.S[q1[u] * SST + pp];
       x[u][3] = W.S[q1[u] * SST + qq];
     }
-    double cq[8], sq[8], y[8][2];
+    double y[8][2];
     int pq[8], qv[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int k = kq0 + 2 * u;
-      cq[u] = W.rc[k];
-      sq[u] = W.rs[k];
-      pair32(k, s, cross, pq[u], qv[u]);
-      y[u][0] = W.Q[iq * SST + pq[u]];
-      y[u][1] = W.Q[iq * SST + qv[u]];
+    for (int u = 0; u < 4; ++u) {
+      pair32(r + 4 * u, s, cross, pq[2 * u], qv[2 * u]);
+      pair32((r + 1 + 4 * u) & 15, s, cross, pq[2 * u + 1], qv[2 * u + 1]);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {  // S <- J^T S J on the 2x2 tile (row pair k, column pair kk)
-      const double lpp = c1[u] * x[u][0] - s1[u] * x[u][2], lpq = c1[u] * x[u][1] - s1[u] * x[u][3];
-      const double lqp = s1[u] * x[u][0] + c1[u] * x[u][2], lqq = s1[u] * x[u][1] + c1[u] * x[u][3];
+    for (int v = 0; v < 8; ++v) {
+      y[v][0] = W.Q[iq * SST + pq[v]];
+      y[v][1] = W.Q[iq * SST + qv[v]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {  // S <- J^T S J on the 2x2 tile (row pair r + 4u, column pair kk)
+      const double c1 = ca[u], s1 = sa[u];
+      const double lpp = c1 * x[u][0] - s1 * x[u][2], lpq = c1 * x[u][1] - s1 * x[u][3];
+      const double lqp = s1 * x[u][0] + c1 * x[u][2], lqq = s1 * x[u][1] + c1 * x[u][3];
       W.S[p1[u] * SST + pp] = c2 * lpp - s2 * lpq;
       W.S[p1[u] * SST + qq] = s2 * lpp + c2 * lpq;
       W.S[q1[u] * SST + pp] = c2 * lqp - s2 * lqq;
       W.S[q1[u] * SST + qq] = s2 * lqp + c2 * lqq;
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {  // Q <- Q J: columns p, q of row iq
-      W.Q[iq * SST + pq[u]] = cq[u] * y[u][0] - sq[u] * y[u][1];
-      W.Q[iq * SST + qv[u]] = sq[u] * y[u][0] + cq[u] * y[u][1];
+    for (int v = 0; v < 8; ++v) {  // Q <- Q J: columns p, q of row iq
+      const double cq = (v & 1) ? cb[v >> 1] : ca[v >> 1], sq = (v & 1) ? sb[v >> 1] : sa[v >> 1];
+      W.Q[iq * SST + pq[v]] = cq * y[v][0] - sq * y[v][1];
+      W.Q[iq * SST + qv[v]] = sq * y[v][0] + cq * y[v][1];
     }
     wave_sync();
   }
