@@ -36,6 +36,11 @@
 #include "common.h"
 #include "../../include/dxtb_hip.h"
 
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 namespace {
 
 constexpr int JB = 256;                // threads per workgroup of the scalar variant
@@ -446,6 +451,35 @@ __device__ __forceinline__ BjMat bj_mat(int b, int nmax, const int* nsize, doubl
   return M;
 }
 
+// Data pointers and batch size of one dxtb_jacobi_eigh call, written into the
+// head of the workspace by k_bj_args before the call's other kernels run.
+// The blocked kernels read them from there, so their launch arguments depend
+// only on the workspace address and the call's sizes and sweep counts: the
+// launch sequence of one such key can be replayed as a graph with other
+// matrices.  Grids span a batch capacity (nbcap >= nbatch); workgroups of the
+// rows beyond nbatch return at once.
+struct JArgs {
+  const int* nsize;
+  double* A;
+  double* VT;
+  double* evals;
+  const double* tol_b;
+  int* sweeps;
+  int nbatch;
+  int pad;
+};
+constexpr size_t JARGS_BYTES = 256;  // head of the workspace (keeps the rest aligned)
+
+// The blocks are read through the constant address space: the compiler may
+// then load a field again where it would otherwise keep it in registers
+// across the sweep loops (the block does not change while a kernel runs).
+typedef const __attribute__((address_space(4))) JArgs* JArgsC;
+__device__ __forceinline__ JArgsC jconst(const JArgs* ja) { return (JArgsC)ja; }
+
+__global__ void k_bj_args(JArgs* __restrict__ ja, JArgs v) {
+  if (threadIdx.x == 0) *ja = v;
+}
+
 // Inner phase of block pair k at outer step s, by one wavefront: stage the
 // 32x32 subproblem, sweep it (all pairs at step 0, where every block has a
 // partner; the pairs across the two blocks afterwards), write the diagonal
@@ -620,18 +654,19 @@ __device__ void bj_norms(const BjMat& M, double* red, double& off, double& amax)
 // One workgroup per matrix, all sweeps.  init_v = 0 continues from the VT
 // the split kernels left; sweep_base is then their per-matrix state, whose
 // fourth entry counts the sweeps already done.
-__global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const int* __restrict__ nsize, double* __restrict__ Ag,
-                                                       double* __restrict__ VT, double* __restrict__ evals,
+__global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const JArgs* __restrict__ ja,
                                                        double* __restrict__ work, int max_sweeps, double tol,
-                                                       const double* __restrict__ tol_b, int init_v,
-                                                       const double* __restrict__ sweep_base,
-                                                       int* __restrict__ sweeps_out) {
+                                                       int init_v, const double* __restrict__ sweep_base) {
   extern __shared__ double lds_raw[];
   BlockedLds& L = *reinterpret_cast<BlockedLds*>(lds_raw);
   const int b = blockIdx.x;
+  const JArgsC jc = jconst(ja);
+  if (b >= jc->nbatch) return;
+  const double* __restrict__ tol_b = jc->tol_b;
+  int* __restrict__ sweeps_out = jc->sweeps;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const BjMat M = bj_mat(b, nmax, nsize, Ag, VT, work);
-  double* eb = evals + (size_t)b * nmax;
+  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work);
+  double* eb = jc->evals + (size_t)b * nmax;
   const int base = sweep_base ? (int)sweep_base[4 * b + 3] : 0;
 
   if (init_v) {
@@ -681,9 +716,11 @@ __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const int* __r
 
 constexpr int SPLIT_MIN_N = 128;  // auto: split form from this order, for batches below half of it
 
-__global__ __launch_bounds__(JBB) void k_bj_init(int nmax, double* __restrict__ VT, double* __restrict__ sst) {
+__global__ __launch_bounds__(JBB) void k_bj_init(int nmax, const JArgs* __restrict__ ja, double* __restrict__ sst) {
   const int b = blockIdx.x;
-  double* Vb = VT + (size_t)b * nmax * nmax;
+  const JArgsC jc = jconst(ja);
+  if (b >= jc->nbatch) return;
+  double* Vb = jc->VT + (size_t)b * nmax * nmax;
   for (int e = threadIdx.x; e < nmax * nmax; e += JBB) {
     const int i = e / nmax, j = e - i * nmax;
     Vb[e] = (i == j) ? 1.0 : 0.0;
@@ -691,14 +728,14 @@ __global__ __launch_bounds__(JBB) void k_bj_init(int nmax, double* __restrict__ 
   if (threadIdx.x < 4) sst[4 * b + threadIdx.x] = 0.0;
 }
 
-__global__ __launch_bounds__(JBB) void k_bj_check(int nmax, const int* __restrict__ nsize, double* __restrict__ Ag,
-                                                 double* __restrict__ VT, double* __restrict__ work, double tol,
-                                                 const double* __restrict__ tol_b, double* __restrict__ sst,
-                                                 int sweep) {
+__global__ __launch_bounds__(JBB) void k_bj_check(int nmax, const JArgs* __restrict__ ja, double* __restrict__ work,
+                                                 double tol, double* __restrict__ sst, int sweep) {
   __shared__ double red[NWB];
   const int b = blockIdx.x;
-  if (sst[4 * b + 2] != 0.0) return;
-  const BjMat M = bj_mat(b, nmax, nsize, Ag, VT, work);
+  const JArgsC jc = jconst(ja);
+  if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
+  const double* __restrict__ tol_b = jc->tol_b;
+  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work);
   double off, amax;
   bj_norms(M, red, off, amax);
   const double thr = fmax(1e-300, (tol_b ? tol_b[b] : tol) * fmax(amax, 1.0));
@@ -713,27 +750,27 @@ __global__ __launch_bounds__(JBB) void k_bj_check(int nmax, const int* __restric
   }
 }
 
-__global__ __launch_bounds__(64) void k_bj_inner(int nmax, const int* __restrict__ nsize, double* __restrict__ Ag,
-                                                double* __restrict__ VT, double* __restrict__ work,
+__global__ __launch_bounds__(64) void k_bj_inner(int nmax, const JArgs* __restrict__ ja, double* __restrict__ work,
                                                 const double* __restrict__ sst, int* __restrict__ rot, int s) {
   __shared__ WaveLds W;
   const int b = blockIdx.y, k = blockIdx.x;
-  if (sst[4 * b + 2] != 0.0) return;
-  const BjMat M = bj_mat(b, nmax, nsize, Ag, VT, work);
+  const JArgsC jc = jconst(ja);
+  if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
+  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work);
   if (k >= M.npairs || s >= M.nb - 1) return;
   const bool any = bj_inner_pair(W, M, k, s, sst[4 * b + 1], threadIdx.x);
   if (threadIdx.x == 0) rot[b * KMAX + k] = any ? 1 : 0;
 }
 
-__global__ __launch_bounds__(JBB) void k_bj_product(int nmax, const int* __restrict__ nsize, double* __restrict__ Ag,
-                                                   double* __restrict__ VT, double* __restrict__ work,
+__global__ __launch_bounds__(JBB) void k_bj_product(int nmax, const JArgs* __restrict__ ja, double* __restrict__ work,
                                                    const double* __restrict__ sst, const int* __restrict__ rot,
                                                    int s) {
   extern __shared__ double lds_raw[];
   WaveLds* Ws = reinterpret_cast<WaveLds*>(lds_raw);
   const int b = blockIdx.y;
-  if (sst[4 * b + 2] != 0.0) return;
-  const BjMat M = bj_mat(b, nmax, nsize, Ag, VT, work);
+  const JArgsC jc = jconst(ja);
+  if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
+  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work);
   if (s >= M.nb - 1) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int item = blockIdx.x * NWB + wv;
@@ -742,11 +779,117 @@ __global__ __launch_bounds__(JBB) void k_bj_product(int nmax, const int* __restr
 
 bool attr_done[64];
 
-// workspace: rotations (nbatch x npad x 32 doubles), split state
-// (nbatch x 4 doubles), rotation flags (nbatch x KMAX ints)
-size_t rot_bytes(int nbatch, int nmax) {
+// Batch capacity of a call: the grids of the blocked kernels span it (rows
+// beyond nbatch return at once), so that windows of nearby sizes share one
+// recorded launch sequence.  Powers of two up to 8, then multiples of 8 up to
+// 64, then multiples of 32.
+int batch_capacity(int nbatch) {
+  if (nbatch <= 8) {
+    int c = 1;
+    while (c < nbatch) c <<= 1;
+    return c;
+  }
+  if (nbatch <= 64) return (nbatch + 7) / 8 * 8;
+  return (nbatch + 31) / 32 * 32;
+}
+
+// workspace: argument block (JARGS_BYTES), rotations (nbcap x npad x 32
+// doubles), split state (nbcap x 4 doubles), rotation flags (nbcap x KMAX ints)
+size_t rot_bytes(int nbcap, int nmax) {
   const size_t npad = (size_t)(nmax + SB - 1) / SB * SB;
-  return (size_t)nbatch * npad * SB * sizeof(double);
+  return (size_t)nbcap * npad * SB * sizeof(double);
+}
+
+struct Layout {
+  JArgs* ja;
+  double* rot_q;  // rotations of the current outer step
+  double* sst;    // split state
+  int* rot;       // rotation flags
+};
+
+Layout layout(double* work, int nbcap, int nmax) {
+  Layout w;
+  char* p = reinterpret_cast<char*>(work);
+  w.ja = reinterpret_cast<JArgs*>(p);
+  w.rot_q = reinterpret_cast<double*>(p + JARGS_BYTES);
+  w.sst = w.rot_q + rot_bytes(nbcap, nmax) / sizeof(double);
+  w.rot = reinterpret_cast<int*>(w.sst + 4 * (size_t)nbcap);
+  return w;
+}
+
+// The blocked kernels of one call (after k_bj_args): nsw split sweeps, then
+// the one-workgroup kernel for the remaining max_sweeps (and the eigenvalues).
+// Every argument is fixed by (work, nbcap, nmax, nsw, max_sweeps, tol).
+hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax, int nsw, int max_sweeps, double tol) {
+  const int npairs = (nmax + SB - 1) / SB, nb = 2 * npairs;
+  const int nwork = npairs * (npairs - 1) / 2 + npairs * npairs;
+  if (nsw > 0) {
+    hipLaunchKernelGGL(k_bj_init, dim3((unsigned)nbcap), dim3(JBB), 0, st, nmax, w.ja, w.sst);
+    for (int sw = 0; sw < nsw; ++sw) {
+      hipLaunchKernelGGL(k_bj_check, dim3((unsigned)nbcap), dim3(JBB), 0, st, nmax, w.ja, w.rot_q, tol, w.sst, sw);
+      for (int s = 0; s < nb - 1; ++s) {
+        hipLaunchKernelGGL(k_bj_inner, dim3((unsigned)npairs, (unsigned)nbcap), dim3(64), 0, st, nmax, w.ja,
+                           w.rot_q, w.sst, w.rot, s);
+        hipLaunchKernelGGL(k_bj_product, dim3((unsigned)((nwork + NWB - 1) / NWB), (unsigned)nbcap), dim3(JBB),
+                           NWB * sizeof(WaveLds), st, nmax, w.ja, w.rot_q, w.sst, w.rot, s);
+      }
+    }
+  }
+  hipLaunchKernelGGL(k_jacobi_blocked, dim3((unsigned)nbcap), dim3(JBB), sizeof(BlockedLds), st, nmax, w.ja, w.rot_q,
+                     max_sweeps - nsw, tol, nsw > 0 ? 0 : 1, nsw > 0 ? w.sst : nullptr);
+  return hipGetLastError();
+}
+
+// Recorded launch sequences of the split form, one per (device, workspace,
+// nbcap, nmax, nsw, max_sweeps, tol).  A split call is ~2 nmax/16 launches
+// per sweep; replaying them as one graph costs the host one launch.  The
+// cache only grows (up to GRAPH_CACHE_MAX entries; beyond that new keys are
+// launched directly), so no graph is destroyed while a stream may run it.
+struct GraphKey {
+  int dev, nbcap, nmax, nsw, max_sweeps;
+  double tol;
+  const double* work;
+  bool operator<(const GraphKey& o) const {
+    return std::tie(dev, nbcap, nmax, nsw, max_sweeps, tol, work) <
+           std::tie(o.dev, o.nbcap, o.nmax, o.nsw, o.max_sweeps, o.tol, o.work);
+  }
+};
+constexpr size_t GRAPH_CACHE_MAX = 1024;
+std::mutex graph_mu;
+std::map<GraphKey, hipGraphExec_t> graph_cache;
+
+bool graphs_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("DXTB_JACOBI_GRAPHS");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// Replays (recording on first use) the split sequence of key k on st.
+// Returns hipSuccess, or an error with nothing left capturing on st.
+hipError_t launch_graph(hipStream_t st, const GraphKey& k, const Layout& w) {
+  std::lock_guard<std::mutex> lock(graph_mu);
+  auto it = graph_cache.find(k);
+  if (it == graph_cache.end()) {
+    if (graph_cache.size() >= GRAPH_CACHE_MAX)
+      return enqueue_blocked(st, w, k.nbcap, k.nmax, k.nsw, k.max_sweeps, k.tol);
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    const hipError_t el = enqueue_blocked(st, w, k.nbcap, k.nmax, k.nsw, k.max_sweeps, k.tol);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(st, &g);
+    if (el != hipSuccess || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return el != hipSuccess ? el : e;
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return e;
+    it = graph_cache.emplace(k, x).first;
+  }
+  return hipGraphLaunch(it->second, st);
 }
 
 }  // namespace
@@ -755,7 +898,9 @@ extern "C" int dxtb_jacobi_max_n(void) { return JMAX; }
 
 extern "C" size_t dxtb_jacobi_work_size(int nbatch, int nmax) {
   if (nbatch <= 0 || nmax <= 0) return 0;
-  return rot_bytes(nbatch, nmax) + (size_t)nbatch * 4 * sizeof(double) + (size_t)nbatch * KMAX * sizeof(int);
+  const int nbcap = batch_capacity(nbatch);
+  return JARGS_BYTES + rot_bytes(nbcap, nmax) + (size_t)nbcap * 4 * sizeof(double) +
+         (size_t)nbcap * KMAX * sizeof(int);
 }
 
 extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals,
@@ -787,30 +932,21 @@ extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* 
   // split form for a few large matrices: one workgroup per matrix would
   // leave most compute units idle and run the product items of a step one
   // after another
-  const int nsw = min(max_sweeps, split_sweeps);
+  int nsw = min(max_sweeps, split_sweeps);
   const bool split = nsw > 0 && (variant == 3 || (variant == 0 && nmax >= SPLIT_MIN_N && 2 * nbatch < nmax));
-  const double* base = nullptr;
-  if (split) {
-    double* sst = work + rot_bytes(nbatch, nmax) / sizeof(double);
-    int* rot = reinterpret_cast<int*>(sst + 4 * nbatch);
-    const int npairs = (nmax + SB - 1) / SB, nb = 2 * npairs;
-    const int nwork = npairs * (npairs - 1) / 2 + npairs * npairs;
-    hipLaunchKernelGGL(k_bj_init, dim3((unsigned)nbatch), dim3(JBB), 0, st, nmax, VT, sst);
-    for (int sw = 0; sw < nsw; ++sw) {
-      hipLaunchKernelGGL(k_bj_check, dim3((unsigned)nbatch), dim3(JBB), 0, st, nmax, nsize, A, VT, work, tol, tol_b,
-                         sst, sw);
-      for (int s = 0; s < nb - 1; ++s) {
-        hipLaunchKernelGGL(k_bj_inner, dim3((unsigned)npairs, (unsigned)nbatch), dim3(64), 0, st, nmax, nsize, A, VT,
-                           work, sst, rot, s);
-        hipLaunchKernelGGL(k_bj_product, dim3((unsigned)((nwork + NWB - 1) / NWB), (unsigned)nbatch), dim3(JBB),
-                           NWB * sizeof(WaveLds), st, nmax, nsize, A, VT, work, sst, rot, s);
-      }
-    }
-    DXTB_CHECK(hipGetLastError());
-    base = sst;
-    max_sweeps -= nsw;
+  if (!split) nsw = 0;
+  const int nbcap = batch_capacity(nbatch);
+  const Layout w = layout(work, nbcap, nmax);
+  const JArgs args{nsize, A, VT, evals, tol_b, sweeps, nbatch, 0};
+  hipLaunchKernelGGL(k_bj_args, dim3(1), dim3(64), 0, st, w.ja, args);
+  DXTB_CHECK(hipGetLastError());
+  // the recorded sequence needs a stream that can be captured: not the null
+  // stream, and not one that the caller is capturing already
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (split && st != nullptr && graphs_enabled()) DXTB_CHECK(hipStreamIsCapturing(st, &cs));
+  if (split && st != nullptr && graphs_enabled() && cs == hipStreamCaptureStatusNone) {
+    const GraphKey k{dev, nbcap, nmax, nsw, max_sweeps, tol, work};
+    return (int)launch_graph(st, k, w);
   }
-  hipLaunchKernelGGL(k_jacobi_blocked, dim3((unsigned)nbatch), dim3(JBB), sizeof(BlockedLds), st, nmax, nsize, A, VT,
-                     evals, work, max_sweeps, tol, tol_b, split ? 0 : 1, base, sweeps);
-  return (int)hipGetLastError();
+  return (int)enqueue_blocked(st, w, nbcap, nmax, nsw, max_sweeps, tol);
 }

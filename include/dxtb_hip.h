@@ -210,7 +210,9 @@ int dxtb_gb_forces(int nbatch, int nat_max, const int* nat, const int* numbers, 
 /* Largest matrix order handled by dxtb_jacobi_eigh. */
 int dxtb_jacobi_max_n(void);
 
-/* Bytes of the workspace dxtb_jacobi_eigh needs for the blocked variants. */
+/* Bytes of the workspace dxtb_jacobi_eigh needs for the blocked variants
+ * (sized for a batch capacity: nbatch rounded up to a power of two up to 8,
+ * a multiple of 8 up to 64, else a multiple of 32). */
 size_t dxtb_jacobi_work_size(int nbatch, int nmax);
 
 /* Batched symmetric eigensolver, cyclic two-sided Jacobi.  variant 1: scalar
@@ -231,7 +233,11 @@ size_t dxtb_jacobi_work_size(int nbatch, int nmax);
  * max_sweeps, when max|offdiag| <= tol * max(1, max|diag|), or when a sweep
  * no longer halves max|offdiag| within 1000x of that bar (rounding floor);
  * tol_b (nbatch, device, may be NULL) replaces tol per matrix; sweeps
- * (nbatch) may be NULL.  Replaces the per-iteration eigensolve of
+ * (nbatch) may be NULL.  On a stream other than the null stream that is not
+ * being captured, the split launches are recorded once per (workspace, batch
+ * capacity, nmax, split sweeps, max_sweeps, tol) and replayed as one graph
+ * (environment DXTB_JACOBI_GRAPHS=0: direct launches); calls that share a
+ * workspace must be ordered (one stream).  Replaces the per-iteration eigensolve of
  * scf/pure/ovlp_diag.py:36-62 (xitorch exacteig / torch.linalg.eigh). */
 int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals, double* work,
                      int variant, int max_sweeps, int split_sweeps, double tol, const double* tol_b, int* sweeps,

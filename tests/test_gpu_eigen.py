@@ -132,3 +132,64 @@ def test_split_few_sweeps(cuda):
         e3, _, sw3 = _jacobi(A, [n], 3, sweeps=ms)
         assert int(sw2[0]) == ms and int(sw3[0]) == ms
         torch.testing.assert_close(e3, e2, atol=1e-12, rtol=0)
+
+
+def _jacobi_into(lib, A, nsize, work, split, sweeps=50, tol=1e-14):
+    """One split-form call on the current stream with a caller-held workspace
+    (A is solved in place)."""
+    B, n, _ = A.shape
+    V = torch.empty_like(A)
+    e = torch.empty((B, n), dtype=A.dtype, device=A.device)
+    sw = torch.zeros(B, dtype=torch.int32, device=A.device)
+    ns = torch.as_tensor(nsize, dtype=torch.int32, device=A.device)
+    rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(ns), _lib.ptr(A), _lib.ptr(V), _lib.ptr(e), _lib.ptr(work), 3, sweeps,
+                              split, tol, None, _lib.ptr(sw), _lib.stream_ptr(A.device))
+    _lib.check(rc, "dxtb_jacobi_eigh")
+    return e, V, sw
+
+
+def test_split_graph_replay_matches_direct(cuda):
+    """On a side stream the split form's launches are recorded once per
+    (workspace, batch capacity, order, sweep counts) and replayed: the second
+    call here replays the first call's record with other buffers and another
+    batch size of the same capacity (5 and 7 matrices, capacity 8).  Both
+    give the results of the same calls launched directly (default stream),
+    bit for bit, including the sweep counts and a matrix that converges
+    before the split sweeps end."""
+    lib = _lib.load()
+    assert torch.cuda.current_stream(cuda).cuda_stream == 0  # the default stream: direct launches
+    SPLIT = 5
+    nmax = 160
+    g = torch.Generator().manual_seed(5)
+    batches = []
+    for sizes in ([160, 150, 97, 33, 160], [120, 160, 160, 5, 160, 64, 159]):
+        A = torch.zeros(len(sizes), nmax, nmax, dtype=torch.float64)
+        for b, n in enumerate(sizes):
+            M = torch.randn(n, n, generator=g, dtype=torch.float64)
+            d = torch.sort(torch.randn(n, generator=g, dtype=torch.float64)).values
+            # near-diagonal (a warm start) except the first matrix
+            A[b, :n, :n] = 0.5 * (M + M.T) if b == 0 else torch.diag(d) + 1e-3 * (M + M.T)
+        batches.append((A.to(cuda), sizes))
+    wsize = lib.dxtb_jacobi_work_size(8, nmax) // 8
+    direct = []
+    wd = torch.empty(wsize, dtype=torch.float64, device=cuda)
+    for A, sizes in batches:
+        direct.append(_jacobi_into(lib, A.clone(), sizes, wd, split=SPLIT))
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream(device=cuda)
+    wg = torch.empty(wsize, dtype=torch.float64, device=cuda)
+    with torch.cuda.stream(side):
+        assert torch.cuda.current_stream(cuda).cuda_stream != 0
+        graphed = [_jacobi_into(lib, A.clone(), sizes, wg, split=SPLIT) for A, sizes in batches]
+    torch.cuda.synchronize()
+    for (e0, V0, s0), (e1, V1, s1), (A, sizes) in zip(direct, graphed, batches):
+        assert s0.tolist() == s1.tolist()
+        # sweeps left to the finisher, and stops within the split sweeps
+        assert max(s0.tolist()) > SPLIT and min(s0.tolist()) <= SPLIT
+        torch.testing.assert_close(e1, e0, atol=0, rtol=0)
+        torch.testing.assert_close(V1, V0, atol=0, rtol=0)
+        # and the eigenpairs are right
+        for b, n in enumerate(sizes):
+            a, v = A[b, :n, :n], V1[b, :n, :n].T
+            scale = max(1.0, e1[b, :n].abs().max().item())
+            torch.testing.assert_close(a @ v, v * e1[b, :n], atol=16 * n * 2.2e-16 * scale, rtol=0)
