@@ -13,7 +13,14 @@ Mirrors the reference calculator API (``src/dxtb/_src/calculators/gfn1.py:38-70`
   like the reference, it requires ``positions.requires_grad`` (RuntimeError
   otherwise); the forces come from the hand-written backward kernels;
 * results are cached by tensor identity of the inputs; ``reset()`` clears
-  the cache (``types/base.py:758-763``).
+  the cache (``types/base.py:758-763``);
+* ``opts`` takes the reference's SCF keys: ``scf_mode`` "implicit"
+  (broyden1 on the residual, the default here) or "full" (the reference's
+  default: the full-mode loop with ``mixer`` "anderson", or "simple";
+  "broyden" falls back to Anderson with a warning, as in
+  ``scf/unrolling/base.py:81-98``), ``maxiter``, ``x_atol``, ``f_atol``,
+  ``x_atol_max``, ``damp``, ``damp_init``, ``damp_generations``,
+  ``damp_soft_start``, ``fermi_etemp``.
 
 Dispersion (D3/D4) is not part of this path: the energy is the reference's
 total energy with ``exclude=["disp"]`` (``types/base.py:612-663``).
@@ -47,26 +54,78 @@ class SCFConvergenceError(RuntimeError):
     pass
 
 
+# SCF mode and mixer names and integer codes (constants/labels/scf.py)
+_MODE_IMPLICIT, _MODE_FULL = ("default", "implicit"), ("full", "full_tracking", "unrolling")
+_MODE_REMOVED = ("implicit_old", "implicit_nonpure", "nonpure", "non-pure", "old")
+_MIXERS = {0: "simple", 1: "anderson", 2: "broyden", "linear": "simple", "l": "simple", "simple": "simple",
+           "s": "simple", "anderson": "anderson", "a": "anderson", "broyden": "broyden", "b": "broyden"}
+
+
+def _scf_mode(val) -> str:
+    """``calculators/config/scf.py:214-264``: "implicit" or "full"."""
+    if isinstance(val, bool) or not isinstance(val, (str, int)):
+        raise TypeError(f"The scf_mode must be of type 'int' or 'str', but '{type(val)}' was given.")
+    if isinstance(val, int):
+        if val == 2:
+            raise ValueError("The SCF mode with integer code 2 (non-pure implicit) was removed. Use 'implicit'.")
+        if val in (0, 1):
+            return "full" if val == 0 else "implicit"
+    else:
+        v = val.casefold()
+        if v in _MODE_IMPLICIT:
+            return "implicit"
+        if v in _MODE_FULL:
+            return "full"
+        if v in _MODE_REMOVED:
+            raise ValueError(f"The SCF mode '{val}' was removed. Use 'implicit'.")
+        if v in ("single-shot", "single_shot", "experimental"):
+            raise NotImplementedError(f"The SCF mode '{val}' is not part of this path.")
+    raise ValueError(f"Unknown SCF mode '{val}'. Use one of 'implicit', 'full'.")
+
+
 def _opts_to_scf(opts: dict | None) -> _scf.SCFOptions:
     o = _scf.SCFOptions()
     if not opts:
         return o
+    mixer = None
     for key, val in opts.items():
         if key in ("f_atol", "f_tol"):
             o.f_tol = float(val)
         elif key in ("x_atol", "x_tol"):
             o.x_tol = float(val)
+        elif key in ("x_atol_max", "x_tol_max"):
+            o.x_tol_max = float(val)
         elif key == "maxiter":
             o.maxiter = int(val)
         elif key in ("fermi_etemp", "etemp"):
             o.etemp = float(val)
         elif key == "damp":
             o.damp = float(val)
-        elif key in ("verbosity", "scf_mode", "scp_mode", "mixer", "int_driver", "batch_mode",
-                     "force_convergence", "exclude"):
+        elif key == "damp_init":
+            o.damp_init = float(val)
+        elif key == "damp_generations":
+            o.damp_generations = int(val)
+        elif key == "damp_soft_start":
+            o.damp_soft_start = bool(val)
+        elif key == "scf_mode":
+            o.scf_mode = _scf_mode(val)
+        elif key == "mixer":
+            m = _MIXERS.get(val.casefold() if isinstance(val, str) else val)
+            if m is None:
+                raise ValueError(f"Unknown mixer '{val}'.")
+            mixer = m
+        elif key in ("verbosity", "scp_mode", "int_driver", "batch_mode", "force_convergence", "exclude"):
             continue
         else:
             raise ValueError(f"Unknown option '{key}'.")
+    if o.scf_mode == "full":
+        # scf/unrolling/base.py:81-98: no Broyden in the full mode
+        if mixer in (None, "broyden"):
+            if mixer == "broyden":
+                warnings.warn("Broyden mixer is not implemented for SCF with full gradient tracking. "
+                              "Using Anderson mixer instead.", UserWarning, stacklevel=3)
+            mixer = "anderson"
+        o.mixer = mixer
     return o
 
 

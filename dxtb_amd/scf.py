@@ -81,6 +81,22 @@ class SCFOptions:
     # pools' kernels: config-2 batch 132.7 molecules/s against 117.6 with 2
     # and 116.7 with 0
     jacobi_variant: int = 3
+    # fixed-point form (scf/unrolling/base.py, scf/implicit/default.py):
+    # "implicit", broyden1 on the residual; "full", the full-mode loop with
+    # the Anderson or simple mixer (the reference's default mode: its
+    # Broyden is not available there and falls back to Anderson)
+    scf_mode: str = "implicit"
+    mixer: str = "anderson"
+    # full mode: converged when the unmixed residual has L2 norm < x_tol and
+    # max norm < x_tol_max; mixer parameters (constants/defaults.py:148-172)
+    x_tol_max: float = 1e-5
+    damp_init: float = 0.1
+    damp_generations: int = 5
+    damp_soft_start: bool = True
+    damp_diagonal_offset: float = 0.01
+
+
+FULL_MODES = ("full", "full_tracking", "full-tracking")
 
 
 class _SolveHistory:
@@ -622,11 +638,95 @@ class SCFResult:
     state: GeometryState
 
 
+class BatchedFullMixer:
+    """The reference's full-mode mixers, per molecule of a batch, on the
+    potential vectors with the weighted inner product of the orbital-resolved
+    vector (as BatchedBroyden).
+
+    "anderson" (``scf/mixer/anderson.py:163-317``): the history holds the
+    previous inputs x_j and residuals F_j = g(x_j) - x_j (newest first, up to
+    ``generations``); with n of them, theta solves a theta = b,
+    a_ij = <F0 - Fi, F0 - Fj> (diagonal times 1 + offset^2),
+    b_i = <F0 - Fi, F0>, and the new input is x_bar + damp F_bar with
+    x_bar = x0 + sum_j theta_j (x_j - x0), F_bar = F0 + sum_j theta_j (F_j - F0).
+    The first step (the first ``generations`` with soft start) is simple
+    mixing x0 + damp_init F0.  "simple" (``scf/mixer/simple.py``):
+    x0 + damp F0.  Every molecule starts at the same step (one batch), so
+    the step count is shared; rows a caller no longer iterates get theta = 0."""
+
+    def __init__(self, o: SCFOptions, B: int, L: int, device, w: torch.Tensor):
+        if o.mixer not in ("anderson", "simple"):
+            raise ValueError(f"Unknown mixer '{o.mixer}' for the full-mode SCF.")
+        self.kind = o.mixer
+        self.damp, self.damp_init = o.damp, o.damp_init
+        self.G, self.soft, self.offset = o.damp_generations, o.damp_soft_start, o.damp_diagonal_offset
+        self.w = w
+        self.step = 0
+        self.X = torch.zeros((B, self.G, L), dtype=torch.float64, device=device)  # previous inputs
+        self.F = torch.zeros((B, self.G, L), dtype=torch.float64, device=device)  # their residuals
+        self.nh = 0  # history entries held
+
+    def subset(self, keep: torch.Tensor, w: torch.Tensor) -> None:
+        self.X, self.F, self.w = self.X[keep], self.F[keep], w
+
+    @staticmethod
+    def _solve(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+        """a theta = b for a batch of small symmetric positive definite
+        systems (B, n, n): Gaussian elimination without pivoting, in batched
+        elementwise operations on the device."""
+        a, b = a.clone(), b.clone()
+        n = a.shape[-1]
+        for k in range(n):
+            f = a[:, k + 1 :, k] / a[:, k, k].unsqueeze(-1)
+            a[:, k + 1 :, :] -= f.unsqueeze(-1) * a[:, k, :].unsqueeze(1)
+            b[:, k + 1 :] -= f * b[:, k].unsqueeze(-1)
+        t = torch.empty_like(b)
+        for k in range(n - 1, -1, -1):
+            t[:, k] = (b[:, k] - (a[:, k, k + 1 :] * t[:, k + 1 :]).sum(-1)) / a[:, k, k]
+        return t
+
+    def iter(self, x_new: torch.Tensor, x_old: torch.Tensor, active: torch.Tensor):
+        """Mixed inputs and the residuals x_new - x_old (B, L)."""
+        self.step += 1
+        f0 = x_new - x_old
+        if self.kind == "simple":
+            return x_old + self.damp * f0, f0
+        if self.step > self.G or (self.step > 1 and not self.soft):
+            n = min(self.step - 1, self.G, self.nh)
+            Xp, Fp = self.X[:, :n], self.F[:, :n]
+            df = f0.unsqueeze(1) - Fp  # (B, n, L)
+            dfw = df * self.w.unsqueeze(1)
+            a = torch.bmm(dfw, df.transpose(1, 2))
+            b = torch.bmm(dfw, f0.unsqueeze(-1)).squeeze(-1)
+            if self.offset is not None:
+                a = a + torch.diag_embed(torch.diagonal(a, dim1=-2, dim2=-1)) * self.offset**2
+            eye = torch.eye(n, dtype=a.dtype, device=a.device).expand_as(a)
+            act = active.view(-1, 1, 1)
+            theta = self._solve(torch.where(act, a, eye), torch.where(act[:, :, 0], b, 0.0))
+            th = theta.unsqueeze(-1)
+            xbar = x_old + (th * (Xp - x_old.unsqueeze(1))).sum(1)
+            fbar = f0 + (th * (Fp - f0.unsqueeze(1))).sum(1)
+            xmix = xbar + self.damp * fbar
+        else:
+            xmix = x_old + self.damp_init * f0
+        # shift the history by one and put (x_old, F0) in front
+        self.X = torch.cat([x_old.unsqueeze(1), self.X[:, : self.G - 1]], 1)
+        self.F = torch.cat([f0.unsqueeze(1), self.F[:, : self.G - 1]], 1)
+        self.nh = min(self.nh + 1, self.G)
+        return xmix, f0
+
+
 def run_scf(state: GeometryState, cull: float = 0.75) -> SCFResult:
     """Iterate all molecules of ``state`` to self-consistency.  When the
     active molecules drop below ``cull`` of the current batch, the batch is
-    compacted to the active ones (converged molecules are frozen)."""
+    compacted to the active ones (converged molecules are frozen).
+    ``state.opts.scf_mode`` picks the fixed-point form: implicit (broyden1)
+    or full (Anderson / simple mixing)."""
     o = state.opts
+    if o.scf_mode in FULL_MODES:
+        return _run_scf_full(state, cull)
+    if o.scf_mode not in ("implicit", "default"):
+        raise ValueError(f"Unsupported scf_mode '{o.scf_mode}'.")
     B = state.bb.nbatch
     dev = state.dev
     x_full = torch.zeros((B, state.nvec), dtype=torch.float64, device=dev)
@@ -675,6 +775,53 @@ def run_scf(state: GeometryState, cull: float = 0.75) -> SCFResult:
             mix.C, mix.D, mix.w = mix.C[keep], mix.D[keep], cur.wvec
     if bool(active.any()):
         x_full[idx[active]] = x[active]
+    return finalize(state, x_full, iters, conv)
+
+
+def _run_scf_full(state: GeometryState, cull: float) -> SCFResult:
+    """Full-mode loop of the reference (``scf/unrolling/default.py:71-140``
+    and its batched form ``:142-403``): the map's output for the guess is
+    mixed with the guess, then each output with its input; a molecule is
+    converged when its unmixed residual has (weighted) L2 norm < x_tol and
+    max norm < x_tol_max, and its result is that unmixed output (the last
+    one if it does not converge)."""
+    o = state.opts
+    B = state.bb.nbatch
+    dev = state.dev
+    x_full = torch.zeros((B, state.nvec), dtype=torch.float64, device=dev)
+    idx = torch.arange(B, device=dev)
+    cur = state
+    active = torch.ones(B, dtype=torch.bool, device=dev)
+    mix = BatchedFullMixer(o, B, state.nvec, dev, state.wvec)
+    x0 = torch.zeros((B, state.nvec), dtype=torch.float64, device=dev)
+    q, _ = mix.iter(cur.g(x0), x0, active)
+    iters = np.zeros(B, dtype=np.int64)
+    conv = np.zeros(B, dtype=bool)
+    q_new = q
+    for it in range(o.maxiter):
+        q_new = cur.g(q)
+        qmix, d = mix.iter(q_new, q, active)
+        live = cur.wvec > 0
+        dmax = torch.where(live, d.abs(), 0.0).amax(-1)
+        done = active & (wnorm(d, cur.wvec) < o.x_tol) & (dmax < o.x_tol_max)
+        a_host, d_host, rows = active.cpu().numpy(), done.cpu().numpy(), idx.cpu().numpy()
+        iters[rows[a_host]] = it + 1
+        conv[rows[d_host]] = True
+        if d_host.any():
+            x_full[idx[done]] = q_new[done]
+        active = active & ~done
+        q = torch.where(active.unsqueeze(-1), qmix, q)
+        n_act = int(active.sum())
+        if n_act == 0:
+            break
+        if n_act < cull * cur.bb.nbatch:
+            keep = torch.nonzero(active).squeeze(-1)
+            cur = cur.subset(keep)
+            idx = idx[keep]
+            q, q_new, active = q[keep], q_new[keep], active[keep]
+            mix.subset(keep, cur.wvec)
+    if bool(active.any()):
+        x_full[idx[active]] = q_new[active]
     return finalize(state, x_full, iters, conv)
 
 

@@ -21,6 +21,14 @@ Follows (paths relative to ``src/dxtb/_src``):
   (``exlibs/xitorch/_impls/optimize/root/rootsolver.py:42-261``,
   ``_jacobian.py:51-120, 250-316``), alpha = -0.5, damp = 0.5, stop when
   ``|dx| < x_tol`` and ``|y| < f_tol``
+* full-mode fixed point (the reference's default ``scf_mode="full"``) --
+  ``scf/unrolling/default.py:71-140``: the map's first output is mixed with
+  the guess, then every output with the current input by the Anderson mixer
+  (``scf/mixer/anderson.py:163-317``, the default: Broyden is not available
+  there, ``scf/unrolling/base.py:81-98``) or the simple mixer
+  (``scf/mixer/simple.py``); converged when the unmixed residual has
+  L2 norm < x_tol and max norm < x_tol_max (``scf/mixer/base.py``), and the
+  result is the unmixed output
 * energies -- ``scf/base.py:510-619`` (equal partition of -TS) and
   ``calculators/types/energy.py:336-337``
 * forces -- the reference differentiates the converged SCF with autograd
@@ -249,19 +257,86 @@ def broyden1(fcn, x0, f_tol=1e-10, x_tol=1e-10, maxiter=300, alpha=-0.5, damp=0.
     return x, maxiter, False
 
 
+class FullModeMixer:
+    """The reference's full-mode mixers on one system's vector.
+
+    ``kind="anderson"``: Eyert's Anderson mixing as written in
+    ``scf/mixer/anderson.py:163-317`` -- history of the mixed inputs x_j and
+    their residuals F_j = g(x_j) - x_j (newest first); with n = min(step - 1,
+    generations) previous entries, theta solves a theta = b with
+    a_ij = <F0 - Fi | F0 - Fj> (diagonal times 1 + offset^2) and
+    b_i = <F0 - Fi | F0>; then x_bar = x0 + sum_j theta_j (x_j - x0),
+    F_bar = F0 + sum_j theta_j (F_j - F0) and the new input is
+    x_bar + damp F_bar.  Simple mixing x0 + damp_init F0 for the first step,
+    or for the first ``generations`` steps with ``soft_start``.
+    ``kind="simple"``: x_old + damp (x_new - x_old) (``scf/mixer/simple.py``).
+    Defaults: ``constants/defaults.py:148-172``."""
+
+    def __init__(self, kind="anderson", damp=0.5, damp_init=0.1, generations=5, soft_start=True, offset=0.01):
+        self.kind, self.damp, self.damp_init = kind, damp, damp_init
+        self.generations, self.soft_start, self.offset = generations, soft_start, offset
+        self.step = 0
+        self.hist = []  # previous (input, residual) pairs, newest first, at most `generations`
+        self.delta = None
+
+    def iter(self, x_new, x_old):
+        self.step += 1
+        f0 = x_new - x_old
+        self.delta = f0
+        if self.kind == "simple":
+            return x_old + self.damp * f0
+        if self.step > self.generations or (self.step > 1 and not self.soft_start):
+            prev = self.hist[: min(self.step - 1, self.generations)]
+            df = torch.stack([f0 - fj for _, fj in prev])
+            a = df @ df.T
+            b = df @ f0
+            if self.offset is not None:
+                a = a + torch.diag(torch.diagonal(a)) * self.offset**2
+            theta = torch.linalg.solve(a, b)
+            xbar = x_old + sum(t * (xj - x_old) for t, (xj, _) in zip(theta, prev))
+            fbar = f0 + sum(t * (fj - f0) for t, (_, fj) in zip(theta, prev))
+            xmix = xbar + self.damp * fbar
+        else:
+            xmix = x_old + self.damp_init * f0
+        self.hist = [(x_old, f0)] + self.hist[: self.generations - 1]
+        return xmix
+
+
+def mix_full(g, x0, kind="anderson", x_tol=1e-10, x_tol_max=1e-5, maxiter=300, **mixer_opts):
+    """Full-mode SCF loop (``scf/unrolling/default.py:71-140``) on the map g:
+    returns (unmixed output at convergence, loop iterations, converged)."""
+    mixer = FullModeMixer(kind, **mixer_opts)
+    q = mixer.iter(g(x0), x0)  # the guess is mixed too (MIX_GUESS)
+    q_new = q
+    for it in range(maxiter):
+        q_new = g(q)
+        q = mixer.iter(q_new, q)
+        d = mixer.delta
+        if d.norm() < x_tol and d.abs().max() < x_tol_max:
+            return q_new, it + 1, True
+    return q_new, maxiter, False
+
+
 class Result(dict):
     pass
 
 
-def singlepoint(numbers, positions, gfn, charge=0.0, tol=1e-10, maxiter=300, forces=False, solvation=None):
+def singlepoint(numbers, positions, gfn, charge=0.0, tol=1e-10, maxiter=300, forces=False, solvation=None,
+                scf_mode="implicit", mixer="anderson", x_tol_max=1e-5):
     """Total energy (repulsion + electronic + -TS) and optionally forces;
-    ``solvation`` adds generalized Born implicit solvation (see System)."""
+    ``solvation`` adds generalized Born implicit solvation (see System).
+    ``scf_mode="implicit"``: broyden1 on the residual; ``"full"``: the
+    full-mode loop with ``mixer`` "anderson" or "simple" (x_tol = tol)."""
     sysm = System(numbers, positions, gfn, charge, solvation=solvation)
     with torch.no_grad():
         m = sysm.build(positions)
         n = sysm.nao + (9 * sysm.nat if sysm.is_gfn2 else 0)
         x0 = torch.zeros(n, dtype=torch.float64)
-        xs, niter, conv = broyden1(lambda x: x - sysm.g(m, x), x0, f_tol=tol, x_tol=tol, maxiter=maxiter)
+        if scf_mode == "full":
+            xs, niter, conv = mix_full(lambda x: sysm.g(m, x), x0, mixer, x_tol=tol, x_tol_max=x_tol_max,
+                                       maxiter=maxiter)
+        else:
+            xs, niter, conv = broyden1(lambda x: x - sysm.g(m, x), x0, f_tol=tol, x_tol=tol, maxiter=maxiter)
         vorb, vdp, vqp = sysm.unpack(xs)
         f = sysm.fock(m, vorb, vdp, vqp)
         p, e, c, occ = sysm.solve_density(m, f)

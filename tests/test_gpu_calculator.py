@@ -52,3 +52,15 @@ def test_api_errors(cuda, geometries):
         calc.get_forces(pos)  # requires_grad is False
     with pytest.raises(DtypeError):
         calc.get_energy(pos.float())
+
+
+def test_full_mode_option(cuda, geometries):
+    """opts={"scf_mode": "full"}: the reference's default SCF (Anderson
+    mixing) gives the implicit mode's energy and forces."""
+    m = geometries["MB16_43_01"]
+    nums = torch.tensor(m["numbers"], device=cuda)
+    pos = torch.tensor(m["positions"], dtype=torch.float64, device=cuda, requires_grad=True)
+    full = GFN2Calculator(nums, opts={"scf_mode": "full", "x_atol": 1e-10}, device=cuda)
+    impl = GFN2Calculator(nums, device=cuda)
+    assert abs(full.get_energy(pos).item() - impl.get_energy(pos).item()) < 1e-8
+    torch.testing.assert_close(full.get_forces(pos), impl.get_forces(pos), atol=1e-7, rtol=0)
