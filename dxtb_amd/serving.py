@@ -112,6 +112,9 @@ class SlotPool:
 
     def __init__(self, gfn: str, nao_max: int, capacity: int, device, opts: hscf.SCFOptions, graphs: bool = False,
                  graph_window_max: int | None = None):
+        if opts.scf_mode not in ("implicit", "default"):
+            raise ValueError(f"the serving pools iterate the implicit mode (broyden1), not scf_mode "
+                             f"'{opts.scf_mode}'; the full mode runs through scf.run_scf and the calculators")
         self.gfn = gfn
         self.dims = class_dims(nao_max)
         self.cap = capacity

@@ -6,6 +6,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+from dxtb_amd import scf as hscf
 from dxtb_amd import serving as sv
 from dxtb_amd import workloads
 
@@ -67,3 +68,10 @@ def test_sweep_prediction_by_window_size(monkeypatch):
     big.recent = {0: [3]}
     assert big.predict(2) == 6 and big.fresh_sweeps == 14
     assert [_SolveHistory.size_class(n) for n in (1, 8, 9, 64, 65)] == [0, 0, 1, 1, 2]
+
+
+def test_pools_reject_full_mode():
+    """The pools run broyden1 only; the full mode is refused before any
+    device work."""
+    with pytest.raises(ValueError, match="implicit mode"):
+        sv.SlotPool("gfn2", 48, 8, "cpu", hscf.SCFOptions(scf_mode="full"))
