@@ -14,6 +14,13 @@ Mirrors the reference calculator API (``src/dxtb/_src/calculators/gfn1.py:38-70`
   otherwise); the forces come from the hand-written backward kernels;
 * results are cached by tensor identity of the inputs; ``reset()`` clears
   the cache (``types/base.py:758-763``);
+* the single-point properties of the reference's energy calculator
+  (``types/energy.py:55-65``): ``get_iterations``, ``get_density``,
+  ``get_mo_energies`` (ascending), ``get_occupation``, ``get_coefficients``,
+  ``get_charges`` / ``get_mulliken_charges`` (orbital-resolved),
+  ``get_bond_orders`` (Wiberg), ``get_potential_energy``, and
+  ``get_property(name, ...)`` for each; other properties (Hessian,
+  vibrations, polarizabilities, ...) raise NotImplementedError;
 * ``opts`` takes the reference's SCF keys: ``scf_mode`` "implicit"
   (broyden1 on the residual, the default here) or "full" (the reference's
   default: the full-mode loop with ``mixer`` "anderson", or "simple";
@@ -215,12 +222,83 @@ class _Calculator:
         f = _scf.forces(res)
         return f.reshape(positions.shape)
 
+    # single-point properties (calculators/types/energy.py:55-65, abc.py:54-560)
+    implemented_properties = ["energy", "forces", "bond_orders", "charges", "coefficients", "density",
+                              "iterations", "mo_energies", "occupation"]
+
+    def get_implemented_properties(self) -> list[str]:
+        return list(self.implemented_properties)
+
+    def _out(self, t):
+        return t if self.batched else t[0]
+
+    def get_potential_energy(self, positions, chrg=0, spin=None, **_):
+        return self.get_energy(positions, chrg, spin)
+
+    def get_iterations(self, positions, chrg=0, spin=None, **_):
+        res = self.singlepoint(positions, chrg, spin)
+        return self._out(torch.as_tensor(res.iterations, device=self.device))
+
+    def get_density(self, positions, chrg=0, spin=None, **_):
+        """Density matrix P (.., nao, nao) of the converged SCF."""
+        return self._out(self.singlepoint(positions, chrg, spin).density)
+
+    @staticmethod
+    def _mo_order(res):
+        # the eigensolver leaves the orbitals unsorted; the reference's eigh
+        # returns them by ascending energy, padding orbitals (energy 0) included
+        return torch.argsort(res.emo, dim=-1, stable=True)
+
+    def get_mo_energies(self, positions, chrg=0, spin=None, **_):
+        res = self.singlepoint(positions, chrg, spin)
+        return self._out(torch.gather(res.emo, -1, self._mo_order(res)))
+
+    def get_occupation(self, positions, chrg=0, spin=None, **_):
+        """Fermi occupations (.., 2, nao) of the alpha and beta orbitals, in
+        the order of get_mo_energies."""
+        res = self.singlepoint(positions, chrg, spin)
+        idx = self._mo_order(res).unsqueeze(1).expand_as(res.occupation)
+        return self._out(torch.gather(res.occupation, -1, idx))
+
+    def get_coefficients(self, positions, chrg=0, spin=None, **_):
+        """Orbital coefficients (.., nao, nao), one orbital per column, in the
+        order of get_mo_energies (signs are arbitrary, as for any eigensolver)."""
+        res = self.singlepoint(positions, chrg, spin)
+        idx = self._mo_order(res).unsqueeze(1).expand_as(res.coeffs)
+        return self._out(torch.gather(res.coeffs, -1, idx))
+
+    def get_charges(self, positions, chrg=0, spin=None, **_):
+        """Orbital-resolved Mulliken charges n0 - diag(P S) (.., nao)
+        (``scf/pure/conversions.py:203-252``)."""
+        res = self.singlepoint(positions, chrg, spin)
+        st = res.state
+        return self._out(st.n0 - (res.density * st.S).sum(-1))
+
+    def get_mulliken_charges(self, positions, chrg=0, spin=None, **kw):
+        return self.get_charges(positions, chrg, spin, **kw)
+
+    def get_bond_orders(self, positions, chrg=0, spin=None, **_):
+        """Wiberg bond orders (.., nat, nat): the atom blocks of
+        (P S) * (P S)^T summed, diagonal zero (``wavefunction/wiberg.py:33-60``)."""
+        res = self.singlepoint(positions, chrg, spin)
+        st = res.state
+        PS = res.density @ st.S
+        W = PS * PS.transpose(-1, -2)
+        bb = st.bb
+        live = torch.arange(bb.nao_max, device=W.device)[None, :] < bb.dev["nao"][:, None].long()
+        onehot = torch.nn.functional.one_hot(bb.dev["ao_atom"], bb.nat_max).to(W.dtype) * live.unsqueeze(-1)
+        wbo = onehot.transpose(1, 2) @ W @ onehot
+        wbo.diagonal(dim1=-2, dim2=-1).fill_(0.0)
+        return self._out(wbo)
+
     def get_property(self, name, positions, chrg=0, spin=None, **kw):
-        if name == "energy":
-            return self.get_energy(positions, chrg, spin)
-        if name == "forces":
-            return self.get_forces(positions, chrg, spin, **kw)
-        raise NotImplementedError(f"Property '{name}' is not implemented on the MI355X path.")
+        getters = {"energy": self.get_energy, "forces": self.get_forces, "bond_orders": self.get_bond_orders,
+                   "charges": self.get_charges, "coefficients": self.get_coefficients,
+                   "density": self.get_density, "iterations": self.get_iterations,
+                   "mo_energies": self.get_mo_energies, "occupation": self.get_occupation}
+        if name not in getters:
+            raise NotImplementedError(f"Property '{name}' is not implemented on the MI355X path.")
+        return getters[name](positions, chrg, spin, **kw)
 
 
 class GFN1Calculator(_Calculator):

@@ -64,3 +64,56 @@ def test_full_mode_option(cuda, geometries):
     impl = GFN2Calculator(nums, device=cuda)
     assert abs(full.get_energy(pos).item() - impl.get_energy(pos).item()) < 1e-8
     torch.testing.assert_close(full.get_forces(pos), impl.get_forces(pos), atol=1e-7, rtol=0)
+
+
+def test_scf_properties_vs_oracle(cuda, geometries):
+    """Single-point properties of the reference calculator (density, MO
+    energies in ascending order, occupations, coefficients, orbital-resolved
+    Mulliken charges, iterations, Wiberg bond orders) against the oracle's
+    SCF converged to 1e-11."""
+    m = geometries["MB16_43_01"]
+    nums = torch.tensor(m["numbers"], device=cuda)
+    pos = torch.tensor(m["positions"], dtype=torch.float64, device=cuda)
+    calc = GFN2Calculator(nums, opts={"x_atol": 1e-10, "f_atol": 1e-10}, device=cuda)
+    ref = oscf.singlepoint(m["numbers"], torch.tensor(m["positions"], dtype=torch.float64), "gfn2", tol=1e-11)
+    P = calc.get_density(pos).cpu()
+    torch.testing.assert_close(P, ref["density"], atol=1e-8, rtol=0)
+    e = calc.get_mo_energies(pos).cpu()
+    assert (e[1:] >= e[:-1]).all()
+    torch.testing.assert_close(e, ref["emo"], atol=1e-8, rtol=0)
+    occ = calc.get_occupation(pos).cpu()
+    torch.testing.assert_close(occ, ref["occupation"], atol=1e-8, rtol=0)
+    C = calc.get_coefficients(pos).cpu()
+    S = calc.singlepoint(pos).state.S[0].cpu()
+    torch.testing.assert_close(C.T @ S @ C, torch.eye(C.shape[0], dtype=torch.float64), atol=1e-10, rtol=0)
+    torch.testing.assert_close((C * occ.sum(0)) @ C.T, P, atol=1e-10, rtol=0)
+    q = calc.get_charges(pos).cpu()
+    torch.testing.assert_close(q, ref["charges"][0], atol=1e-8, rtol=0)
+    assert abs(q.sum().item()) < 1e-10  # neutral molecule
+    assert int(calc.get_iterations(pos)) == int(calc.singlepoint(pos).iterations[0]) > 0
+    # Wiberg bond orders by brute force over atom pairs from the oracle's P and S
+    wbo = calc.get_bond_orders(pos).cpu()
+    ao_atom = calc.singlepoint(pos).state.bb.ao_atom[0]
+    PS = ref["density"] @ S
+    nat = len(m["numbers"])
+    brute = torch.zeros(nat, nat, dtype=torch.float64)
+    for a in range(nat):
+        for b in range(nat):
+            if a != b:
+                ia, ib = torch.tensor((ao_atom == a).nonzero()[0]), torch.tensor((ao_atom == b).nonzero()[0])
+                brute[a, b] = (PS[ia][:, ib] * PS[ib][:, ia].T).sum()
+    torch.testing.assert_close(wbo, brute, atol=1e-8, rtol=0)
+    torch.testing.assert_close(calc.get_property("density", pos).cpu(), P, atol=0, rtol=0)
+    with pytest.raises(NotImplementedError):
+        calc.get_property("polarizability", pos)
+
+
+def test_h2_bond_order_is_one(cuda, geometries):
+    """GFN2 H2: two electrons in the bonding orbital of a minimal basis,
+    P = (1 + s)^-1 [[1, 1], [1, 1]], so (P S)_AB = 1 and the Wiberg bond
+    order is exactly one (the 300 K smearing is negligible at this gap)."""
+    m = geometries["H2"]
+    calc = GFN2Calculator(torch.tensor(m["numbers"], device=cuda), device=cuda)
+    pos = torch.tensor(m["positions"], dtype=torch.float64, device=cuda)
+    wbo = calc.get_bond_orders(pos).cpu()
+    assert abs(wbo[0, 1].item() - 1.0) < 1e-8 and wbo[0, 0].item() == 0.0
