@@ -703,6 +703,9 @@ class BatchedFullMixer:
             eye = torch.eye(n, dtype=a.dtype, device=a.device).expand_as(a)
             act = active.view(-1, 1, 1)
             theta = self._solve(torch.where(act, a, eye), torch.where(act[:, :, 0], b, 0.0))
+            if not bool(torch.isfinite(theta).all()):
+                # as torch.linalg.solve in the reference mixer
+                raise RuntimeError("Anderson mixing: singular history (linearly dependent residuals)")
             th = theta.unsqueeze(-1)
             xbar = x_old + (th * (Xp - x_old.unsqueeze(1))).sum(1)
             fbar = f0 + (th * (Fp - f0.unsqueeze(1))).sum(1)

@@ -168,3 +168,17 @@ def test_calculator_scf_options():
         calculators._opts_to_scf({"mixer": "diis"})
     with pytest.raises(NotImplementedError):
         calculators._opts_to_scf({"scf_mode": "single-shot"})
+
+
+def test_batched_mixer_singular_history_raises():
+    """Identical residuals make the Anderson system singular: an error, as
+    torch.linalg.solve raises in the reference mixer."""
+    o = hscf.SCFOptions(mixer="anderson", damp_soft_start=False)
+    w = torch.ones(1, 3, dtype=torch.float64)
+    mix = hscf.BatchedFullMixer(o, 1, 3, torch.device("cpu"), w)
+    x = torch.zeros(1, 3, dtype=torch.float64)
+    f = torch.tensor([[1.0, 2.0, 3.0]], dtype=torch.float64)
+    active = torch.ones(1, dtype=torch.bool)
+    mix.iter(x + f, x, active)
+    with pytest.raises(RuntimeError, match="singular"):
+        mix.iter(x + f, x, active)  # the same residual again
