@@ -245,6 +245,10 @@ struct WaveLds {
 };
 
 constexpr int KMAX = JMAX / SB;  // max block pairs
+// LDS of one product wavefront: only the staging tile (the rotations are read
+// from global memory), so two 8-wavefront product workgroups fit on a CU
+constexpr int PROD_TILE = SB * SST;
+constexpr size_t PROD_LDS = (size_t)NWB * PROD_TILE * sizeof(double);
 
 struct BlockedLds {
   WaveLds w[NWB];
@@ -518,10 +522,11 @@ __device__ bool bj_inner_pair(WaveLds& W, const BjMat& M, int k, int s, double t
   return any;
 }
 
-// Product item `item` of outer step s, by one wavefront (W.S is its staging
-// tile): an off-diagonal tile (k < k2) becomes Q_k^T A_kk2 Q_k2, a 32-column
-// panel of the VT rows of pair k becomes Q_k^T VT.  rot: flags of the pairs.
-__device__ void bj_product_item(WaveLds& W, const BjMat& M, const int* rot, int item, int s, int lane) {
+// Product item `item` of outer step s, by one wavefront (Sx: its 32x33
+// staging tile in LDS): an off-diagonal tile (k < k2) becomes
+// Q_k^T A_kk2 Q_k2, a 32-column panel of the VT rows of pair k becomes
+// Q_k^T VT.  rot: flags of the pairs.
+__device__ void bj_product_item(double* __restrict__ Sx, const BjMat& M, const int* rot, int item, int s, int lane) {
   const int npairs = M.npairs, n = M.n, nmax = M.nmax, nlast = M.nlast;
   const int ntile = npairs * (npairs - 1) / 2;
   const int npanel = (n + SB - 1) / SB;
@@ -561,11 +566,11 @@ __device__ void bj_product_item(WaveLds& W, const BjMat& M, const int* rot, int 
       for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          W.S[(rb * BS + acc_row(lane, i)) * SST + cb * BS + (lane & 15)] = acc[rb][cb][i];
+          Sx[(rb * BS + acc_row(lane, i)) * SST + cb * BS + (lane & 15)] = acc[rb][cb][i];
     wave_sync();
     if (rk2) {
       // U^T = Q_k2^T T^T with U = T Q_k2: the same product, T read transposed
-      auto loadT = [&](int m, int c) -> double { return W.S[c * SST + m]; };
+      auto loadT = [&](int m, int c) -> double { return Sx[c * SST + m]; };
       qt_times(M.Qw + (size_t)k2 * SB * SB, loadT, lane, acc);
 #pragma unroll
       for (int rb = 0; rb < 2; ++rb)
@@ -586,7 +591,7 @@ __device__ void bj_product_item(WaveLds& W, const BjMat& M, const int* rot, int 
         const int r = e >> 5, c = e & 31;
         const int gr = gidx(r, bi, bj), gc = gidx(c, ci, cj);
         if (gr < n && gc < n) {
-          const double v = W.S[r * SST + c];
+          const double v = Sx[r * SST + c];
           A[(size_t)gr * nmax + gc] = v;
           A[(size_t)gc * nmax + gr] = v;
         }
@@ -697,7 +702,7 @@ __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const JArgs* _
       }
       __syncthreads();
       const int nwork = bj_nwork(M);  // product phase: tiles and VT panels
-      for (int item = wv; item < nwork; item += NWB) bj_product_item(W, M, L.rot, item, s, lane);
+      for (int item = wv; item < nwork; item += NWB) bj_product_item(W.S, M, L.rot, item, s, lane);
       __syncthreads();
     }
   }
@@ -765,8 +770,7 @@ __global__ __launch_bounds__(64) void k_bj_inner(int nmax, const JArgs* __restri
 __global__ __launch_bounds__(JBB) void k_bj_product(int nmax, const JArgs* __restrict__ ja, double* __restrict__ work,
                                                    const double* __restrict__ sst, const int* __restrict__ rot,
                                                    int s) {
-  extern __shared__ double lds_raw[];
-  WaveLds* Ws = reinterpret_cast<WaveLds*>(lds_raw);
+  extern __shared__ double lds_raw[];  // one staging tile per wavefront
   const int b = blockIdx.y;
   const JArgsC jc = jconst(ja);
   if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
@@ -774,7 +778,7 @@ __global__ __launch_bounds__(JBB) void k_bj_product(int nmax, const JArgs* __res
   if (s >= M.nb - 1) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int item = blockIdx.x * NWB + wv;
-  if (item < bj_nwork(M)) bj_product_item(Ws[wv], M, rot + b * KMAX, item, s, lane);
+  if (item < bj_nwork(M)) bj_product_item(lds_raw + (size_t)wv * PROD_TILE, M, rot + b * KMAX, item, s, lane);
 }
 
 bool attr_done[64];
@@ -831,7 +835,7 @@ hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax,
         hipLaunchKernelGGL(k_bj_inner, dim3((unsigned)npairs, (unsigned)nbcap), dim3(64), 0, st, nmax, w.ja,
                            w.rot_q, w.sst, w.rot, s);
         hipLaunchKernelGGL(k_bj_product, dim3((unsigned)((nwork + NWB - 1) / NWB), (unsigned)nbcap), dim3(JBB),
-                           NWB * sizeof(WaveLds), st, nmax, w.ja, w.rot_q, w.sst, w.rot, s);
+                           PROD_LDS, st, nmax, w.ja, w.rot_q, w.sst, w.rot, s);
       }
     }
   }
@@ -921,7 +925,7 @@ extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* 
     DXTB_CHECK(hipFuncSetAttribute((const void*)k_jacobi_blocked, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)sizeof(BlockedLds)));
     DXTB_CHECK(hipFuncSetAttribute((const void*)k_bj_product, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(NWB * sizeof(WaveLds))));
+                                   (int)PROD_LDS));
     attr_done[dev] = true;
   }
   if (scalar) {
