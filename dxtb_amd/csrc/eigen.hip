@@ -245,10 +245,12 @@ struct WaveLds {
 };
 
 constexpr int KMAX = JMAX / SB;  // max block pairs
-// LDS of one product wavefront: only the staging tile (the rotations are read
-// from global memory), so two 8-wavefront product workgroups fit on a CU
-constexpr int PROD_TILE = SB * SST;
-constexpr size_t PROD_LDS = (size_t)NWB * PROD_TILE * sizeof(double);
+// Product kernel: 4-wavefront workgroups whose wavefronts hold only their
+// 32x32 staging tile in LDS (the rotations are read from global memory):
+// 32 KB per workgroup, so five fit on a CU (20 wavefronts, the register limit)
+constexpr int PW = 4;               // wavefronts per product workgroup
+constexpr int PROD_TILE = SB * SB;  // doubles per staging tile
+constexpr size_t PROD_LDS = (size_t)PW * PROD_TILE * sizeof(double);
 
 struct BlockedLds {
   WaveLds w[NWB];
@@ -522,10 +524,20 @@ __device__ bool bj_inner_pair(WaveLds& W, const BjMat& M, int k, int s, double t
   return any;
 }
 
-// Product item `item` of outer step s, by one wavefront (Sx: its 32x33
-// staging tile in LDS): an off-diagonal tile (k < k2) becomes
+// Element (r, c) of a 32x32 staging tile stored without padding: the column
+// is XORed with the row, so that a wavefront's accesses along a row and
+// along a column (the transposed read) fall in distinct LDS banks.
+__device__ __forceinline__ int stile(int r, int c) { return r * SB + (c ^ (r & (SB - 1))); }
+
+// Staging tile index: swizzled 32x32 (SWZ) or padded 32x33 rows.
+template <bool SWZ>
+__device__ __forceinline__ int tix(int r, int c) { return SWZ ? stile(r, c) : r * SST + c; }
+
+// Product item `item` of outer step s, by one wavefront (Sx: its staging tile
+// in LDS, indexed by tix<SWZ>): an off-diagonal tile (k < k2) becomes
 // Q_k^T A_kk2 Q_k2, a 32-column panel of the VT rows of pair k becomes
 // Q_k^T VT.  rot: flags of the pairs.
+template <bool SWZ>
 __device__ void bj_product_item(double* __restrict__ Sx, const BjMat& M, const int* rot, int item, int s, int lane) {
   const int npairs = M.npairs, n = M.n, nmax = M.nmax, nlast = M.nlast;
   const int ntile = npairs * (npairs - 1) / 2;
@@ -559,18 +571,18 @@ __device__ void bj_product_item(double* __restrict__ Sx, const BjMat& M, const i
 #pragma unroll
           for (int i = 0; i < 4; ++i) acc[rb][cb][i] = loadA(rb * BS + acc_row(lane, i), cb * BS + (lane & 15));
     }
-    // T = Q_k^T X -> LDS (row-major, stride SST)
+    // T = Q_k^T X -> LDS (swizzled rows)
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
       for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          Sx[(rb * BS + acc_row(lane, i)) * SST + cb * BS + (lane & 15)] = acc[rb][cb][i];
+          Sx[tix<SWZ>(rb * BS + acc_row(lane, i), cb * BS + (lane & 15))] = acc[rb][cb][i];
     wave_sync();
     if (rk2) {
       // U^T = Q_k2^T T^T with U = T Q_k2: the same product, T read transposed
-      auto loadT = [&](int m, int c) -> double { return Sx[c * SST + m]; };
+      auto loadT = [&](int m, int c) -> double { return Sx[tix<SWZ>(c, m)]; };
       qt_times(M.Qw + (size_t)k2 * SB * SB, loadT, lane, acc);
 #pragma unroll
       for (int rb = 0; rb < 2; ++rb)
@@ -591,7 +603,7 @@ __device__ void bj_product_item(double* __restrict__ Sx, const BjMat& M, const i
         const int r = e >> 5, c = e & 31;
         const int gr = gidx(r, bi, bj), gc = gidx(c, ci, cj);
         if (gr < n && gc < n) {
-          const double v = Sx[r * SST + c];
+          const double v = Sx[tix<SWZ>(r, c)];
           A[(size_t)gr * nmax + gc] = v;
           A[(size_t)gc * nmax + gr] = v;
         }
@@ -702,7 +714,7 @@ __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const JArgs* _
       }
       __syncthreads();
       const int nwork = bj_nwork(M);  // product phase: tiles and VT panels
-      for (int item = wv; item < nwork; item += NWB) bj_product_item(W.S, M, L.rot, item, s, lane);
+      for (int item = wv; item < nwork; item += NWB) bj_product_item<false>(W.S, M, L.rot, item, s, lane);
       __syncthreads();
     }
   }
@@ -767,7 +779,7 @@ __global__ __launch_bounds__(64) void k_bj_inner(int nmax, const JArgs* __restri
   if (threadIdx.x == 0) rot[b * KMAX + k] = any ? 1 : 0;
 }
 
-__global__ __launch_bounds__(JBB) void k_bj_product(int nmax, const JArgs* __restrict__ ja, double* __restrict__ work,
+__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(5))) void k_bj_product(int nmax, const JArgs* __restrict__ ja, double* __restrict__ work,
                                                    const double* __restrict__ sst, const int* __restrict__ rot,
                                                    int s) {
   extern __shared__ double lds_raw[];  // one staging tile per wavefront
@@ -777,8 +789,8 @@ __global__ __launch_bounds__(JBB) void k_bj_product(int nmax, const JArgs* __res
   const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work);
   if (s >= M.nb - 1) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int item = blockIdx.x * NWB + wv;
-  if (item < bj_nwork(M)) bj_product_item(lds_raw + (size_t)wv * PROD_TILE, M, rot + b * KMAX, item, s, lane);
+  const int item = blockIdx.x * PW + wv;
+  if (item < bj_nwork(M)) bj_product_item<true>(lds_raw + (size_t)wv * PROD_TILE, M, rot + b * KMAX, item, s, lane);
 }
 
 bool attr_done[64];
@@ -834,7 +846,7 @@ hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax,
       for (int s = 0; s < nb - 1; ++s) {
         hipLaunchKernelGGL(k_bj_inner, dim3((unsigned)npairs, (unsigned)nbcap), dim3(64), 0, st, nmax, w.ja,
                            w.rot_q, w.sst, w.rot, s);
-        hipLaunchKernelGGL(k_bj_product, dim3((unsigned)((nwork + NWB - 1) / NWB), (unsigned)nbcap), dim3(JBB),
+        hipLaunchKernelGGL(k_bj_product, dim3((unsigned)((nwork + PW - 1) / PW), (unsigned)nbcap), dim3(PW * 64),
                            PROD_LDS, st, nmax, w.ja, w.rot_q, w.sst, w.rot, s);
       }
     }
