@@ -48,7 +48,6 @@ constexpr int NWAVE = JB / 64;         // its wavefronts
 constexpr int JBB = 512;               // threads per workgroup of the blocked variant
 constexpr int NWB = JBB / 64;          // its wavefronts (one block pair or product item each)
 constexpr int JMAX_LDS = 128;          // largest matrix of the scalar variant
-constexpr int JSTRIDE = JMAX_LDS + 1;  // padded LDS row stride (doubles)
 constexpr int JMAX = 1536;             // largest matrix overall
 constexpr int JSCALAR_AUTO = 64;       // auto: scalar variant up to this order
 constexpr int BS = 16;                 // block size of the blocked variant
@@ -111,21 +110,30 @@ __device__ __forceinline__ bool stop_sweeps(double off, double off_prev, double 
 
 // ---------------------------------------------------------------- scalar ----
 
+// LDS layout of the scalar variant for matrices up to order ncap =
+// min(nmax, JMAX_LDS): A with an odd row stride (no bank conflicts down a
+// column), then the rotations of one step (cosines, sines, p and q of up to
+// np pairs) and the reduction slots.  Sized per call, so small matrices leave
+// the CU's LDS to other kernels.
+__host__ __device__ constexpr int scalar_stride(int ncap) { return ncap + 1 + (ncap & 1); }
+__host__ __device__ constexpr int scalar_np(int ncap) { return (ncap + 2) / 2; }
+
 __global__ __launch_bounds__(JB) void k_jacobi_scalar(int nmax, const int* __restrict__ nsize,
                                                       const double* __restrict__ Ag, double* __restrict__ VT,
                                                       double* __restrict__ evals, int max_sweeps, double tol,
                                                       const double* __restrict__ tol_b, int* __restrict__ sweeps_out) {
   extern __shared__ double lds[];
-  constexpr int NP = JMAX_LDS / 2;
+  const int ncap = min(nmax, JMAX_LDS);
+  const int ls = scalar_stride(ncap), NP = scalar_np(ncap);
   const int b = blockIdx.x;
-  const int n = min(max(nsize[b], 0), min(nmax, JMAX_LDS));
+  const int n = min(max(nsize[b], 0), ncap);
   const size_t nn = (size_t)nmax * nmax;
   const double* Ab = Ag + (size_t)b * nn;
   double* Vb = VT + (size_t)b * nn;
   double* eb = evals + (size_t)b * nmax;
 
   double* A = lds;
-  double* rc = lds + JMAX_LDS * JSTRIDE;  // cosines (NP)
+  double* rc = lds + ncap * ls;  // cosines (NP)
   double* rs = rc + NP;                    // sines   (NP)
   int* rp = (int*)(rs + NP);               // p index (NP)
   int* rq = rp + NP;                       // q index (NP), -1 for the odd dummy
@@ -133,7 +141,7 @@ __global__ __launch_bounds__(JB) void k_jacobi_scalar(int nmax, const int* __res
 
   for (int e = threadIdx.x; e < n * n; e += JB) {
     const int i = e / n, j = e - i * n;
-    A[i * JSTRIDE + j] = Ab[(size_t)i * nmax + j];
+    A[i * ls + j] = Ab[(size_t)i * nmax + j];
   }
   for (int e = threadIdx.x; e < nmax * nmax; e += JB) {
     const int i = e / nmax, j = e - i * nmax;
@@ -149,7 +157,7 @@ __global__ __launch_bounds__(JB) void k_jacobi_scalar(int nmax, const int* __res
     double off = 0.0, amax = 0.0;
     for (int e = threadIdx.x; e < n * n; e += JB) {
       const int i = e / n, j = e - i * n;
-      const double x = fabs(A[i * JSTRIDE + j]);
+      const double x = fabs(A[i * ls + j]);
       if (i != j) off = fmax(off, x);
       else amax = fmax(amax, x);
     }
@@ -168,7 +176,7 @@ __global__ __launch_bounds__(JB) void k_jacobi_scalar(int nmax, const int* __res
           q = t;
         }
         double c = 1.0, sn = 0.0;
-        if (q < n) rotation(A[p * JSTRIDE + p], A[q * JSTRIDE + q], A[p * JSTRIDE + q], thr, c, sn);
+        if (q < n) rotation(A[p * ls + p], A[q * ls + q], A[p * ls + q], thr, c, sn);
         rc[k] = c;
         rs[k] = sn;
         rp[k] = p;
@@ -181,17 +189,17 @@ __global__ __launch_bounds__(JB) void k_jacobi_scalar(int nmax, const int* __res
         const double c1 = rc[k], s1 = rs[k], c2 = rc[kk], s2 = rs[kk];
         if (s1 == 0.0 && s2 == 0.0) continue;
         const int p = rp[k], q = rq[k], pp = rp[kk], qq = rq[kk];
-        const double a = A[p * JSTRIDE + pp];
-        const double bq = (qq >= 0) ? A[p * JSTRIDE + qq] : 0.0;
-        const double cq = (q >= 0) ? A[q * JSTRIDE + pp] : 0.0;
-        const double d = (q >= 0 && qq >= 0) ? A[q * JSTRIDE + qq] : 0.0;
+        const double a = A[p * ls + pp];
+        const double bq = (qq >= 0) ? A[p * ls + qq] : 0.0;
+        const double cq = (q >= 0) ? A[q * ls + pp] : 0.0;
+        const double d = (q >= 0 && qq >= 0) ? A[q * ls + qq] : 0.0;
         const double lpp = c1 * a - s1 * cq, lpq = c1 * bq - s1 * d;  // left (rows of pair k)
         const double lqp = s1 * a + c1 * cq, lqq = s1 * bq + c1 * d;
-        A[p * JSTRIDE + pp] = c2 * lpp - s2 * lpq;  // right (columns of pair kk)
-        if (qq >= 0) A[p * JSTRIDE + qq] = s2 * lpp + c2 * lpq;
+        A[p * ls + pp] = c2 * lpp - s2 * lpq;  // right (columns of pair kk)
+        if (qq >= 0) A[p * ls + qq] = s2 * lpp + c2 * lpq;
         if (q >= 0) {
-          A[q * JSTRIDE + pp] = c2 * lqp - s2 * lqq;
-          if (qq >= 0) A[q * JSTRIDE + qq] = s2 * lqp + c2 * lqq;
+          A[q * ls + pp] = c2 * lqp - s2 * lqq;
+          if (qq >= 0) A[q * ls + qq] = s2 * lqp + c2 * lqq;
         }
       }
       // VT <- J^T VT (rows p, q).  Four items per thread are loaded before any
@@ -227,13 +235,14 @@ __global__ __launch_bounds__(JB) void k_jacobi_scalar(int nmax, const int* __res
       __syncthreads();
     }
   }
-  for (int i = threadIdx.x; i < nmax; i += JB) eb[i] = (i < n) ? A[i * JSTRIDE + i] : 0.0;
+  for (int i = threadIdx.x; i < nmax; i += JB) eb[i] = (i < n) ? A[i * ls + i] : 0.0;
   if (threadIdx.x == 0 && sweeps_out != nullptr) sweeps_out[b] = sweep;
 }
 
-size_t scalar_shmem() {
-  constexpr size_t np = JMAX_LDS / 2;
-  return (size_t)JMAX_LDS * JSTRIDE * sizeof(double) + np * (2 * sizeof(double) + 2 * sizeof(int)) +
+size_t scalar_shmem(int nmax) {
+  const int ncap = nmax < JMAX_LDS ? nmax : JMAX_LDS;
+  const size_t np = scalar_np(ncap);
+  return (size_t)ncap * scalar_stride(ncap) * sizeof(double) + np * (2 * sizeof(double) + 2 * sizeof(int)) +
          NWAVE * sizeof(double);
 }
 
@@ -933,7 +942,7 @@ extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* 
   if (dev < 0 || dev >= 64) return DXTB_EINVAL;
   if (!attr_done[dev]) {
     DXTB_CHECK(hipFuncSetAttribute((const void*)k_jacobi_scalar, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)scalar_shmem()));
+                                   (int)scalar_shmem(JMAX_LDS)));
     DXTB_CHECK(hipFuncSetAttribute((const void*)k_jacobi_blocked, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)sizeof(BlockedLds)));
     DXTB_CHECK(hipFuncSetAttribute((const void*)k_bj_product, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -941,7 +950,7 @@ extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* 
     attr_done[dev] = true;
   }
   if (scalar) {
-    hipLaunchKernelGGL(k_jacobi_scalar, dim3((unsigned)nbatch), dim3(JB), scalar_shmem(), st, nmax, nsize, A, VT,
+    hipLaunchKernelGGL(k_jacobi_scalar, dim3((unsigned)nbatch), dim3(JB), scalar_shmem(nmax), st, nmax, nsize, A, VT,
                        evals, max_sweeps, tol, tol_b, sweeps);
     return (int)hipGetLastError();
   }
