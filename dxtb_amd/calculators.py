@@ -8,7 +8,9 @@ Mirrors the reference calculator API (``src/dxtb/_src/calculators/gfn1.py:38-70`
 * ``Calculator(numbers, opts=None, device=None, dtype=torch.float64)`` with
   integer ``numbers`` of shape (nat,) or zero-padded (B, nat);
 * ``get_energy(positions, chrg=0, spin=None)`` -> total energy, summed over
-  atoms: (,) or (B,);
+  atoms: (,) or (B,); ``spin`` is the number of unpaired electrons (scalar
+  or one per molecule; None: from the parity of the electron count), split
+  into alpha and beta counts as in ``wavefunction/filling.py:180-259``;
 * ``get_forces(positions, chrg=0, spin=None, grad_mode="autograd")`` -> -dE/dR;
   like the reference, it requires ``positions.requires_grad`` (RuntimeError
   otherwise); the forces come from the hand-written backward kernels;
@@ -186,19 +188,28 @@ class _Calculator:
             return np.full(B, float(chrg))
         return np.asarray(chrg.detach().cpu() if torch.is_tensor(chrg) else chrg, dtype=np.float64).reshape(B)
 
+    def _spins(self, spin):
+        """Unpaired electrons per molecule (None: from the electron count's
+        parity), like the charges a scalar or one value per molecule."""
+        if spin is None:
+            return None
+        B = self.bb.nbatch
+        s = np.asarray(spin.detach().cpu() if torch.is_tensor(spin) else spin, dtype=np.float64)
+        return np.broadcast_to(s, (B,)).copy() if s.ndim == 0 else s.reshape(B)
+
     def _key(self, positions, chrg, spin):
         cid = id(chrg) if torch.is_tensor(chrg) else chrg
-        return (id(positions), positions._version, cid, spin)
+        sid = id(spin) if torch.is_tensor(spin) else spin
+        return (id(positions), positions._version, cid, sid)
 
     def singlepoint(self, positions, chrg=0, spin=None):
         self._check(positions)
         key = self._key(positions, chrg, spin)
         if key in self.cache:
             return self.cache[key]
-        if spin not in (None, 0):
-            raise NotImplementedError("explicit spin is not supported on this path")
         pos = positions.detach().reshape(self.bb.nbatch, self.bb.nat_max, 3)
-        state = _scf.GeometryState(self.bb, pos, self._charges(chrg), self.scf_opts, solvation=self.solvation)
+        state = _scf.GeometryState(self.bb, pos, self._charges(chrg), self.scf_opts, solvation=self.solvation,
+                                   spins=self._spins(spin))
         res = _scf.run_scf(state)
         if not res.converged.all():
             msg = f"SCF did not converge for {int((~res.converged).sum())} molecule(s)."

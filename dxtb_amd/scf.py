@@ -189,11 +189,38 @@ def _orthogonaliser(spad: torch.Tensor, chunk: int = 32) -> torch.Tensor:
     return X
 
 
+def alpha_beta(nel: np.ndarray, uhf=None) -> np.ndarray:
+    """Alpha and beta electron counts (B, 2) of molecules with nel electrons
+    and uhf unpaired electrons (``wavefunction/filling.py:180-259``): uhf must
+    not exceed the rounded electron count and must have its parity; without
+    it (or where it has the other parity) the parity alone decides; the paired
+    electrons are split equally and the unpaired ones are alpha."""
+    nel = np.atleast_1d(np.asarray(nel, dtype=np.float64))
+    nint = np.floor(nel + 0.5)
+    if uhf is None:
+        uhf = np.zeros_like(nel)
+    else:
+        uhf = np.asarray(uhf.cpu() if torch.is_tensor(uhf) else uhf, dtype=np.float64)
+        uhf = np.broadcast_to(uhf, nel.shape) if uhf.ndim == 0 else uhf
+        if uhf.shape != nel.shape:
+            raise RuntimeError(f"Shape mismatch for unpaired electrons ({uhf.shape}) and number of electrons "
+                               f"({nel.shape}).")
+        if (uhf > nint).any():
+            raise ValueError(f"Number of unpaired electrons ({uhf}) larger than number of electrons ({nel}).")
+        if (np.remainder(uhf, 2) != np.remainder(nint, 2)).any():
+            raise ValueError(f"Odd (even) number of unpaired electrons ({uhf}) but even (odd) number of "
+                             f"electrons ({nel}) given.")
+    nuhf = np.where(np.remainder(uhf, 2) == np.remainder(nint, 2), uhf, np.remainder(nint, 2))
+    diff = np.minimum(nuhf, nel)
+    nb = (nel - diff) / 2.0
+    return np.stack([nb + diff, nb], 1).astype(np.float64)
+
+
 class GeometryState:
     """Everything that depends on the geometry but not on the density."""
 
     def __init__(self, bb: BatchBasis, positions: torch.Tensor, charges=None, opts: SCFOptions | None = None,
-                 solvation=None):
+                 solvation=None, spins=None):
         self.bb = bb
         self.opts = opts or SCFOptions()
         self.gfn = bb.gfn
@@ -237,12 +264,7 @@ class GeometryState:
         chg = np.zeros(B) if charges is None else np.asarray(
             charges.cpu() if torch.is_tensor(charges) else charges, dtype=np.float64).reshape(B)
         nel = nel - chg
-        # alpha/beta split without explicit spin (wavefunction/filling.py:180-259)
-        nint = np.floor(nel + 0.5)
-        nuhf = np.remainder(nint, 2)
-        diff = np.minimum(nuhf, nel)
-        nb = (nel - diff) / 2.0
-        self.nab = to_device(np.stack([nb + diff, nb], 1).astype(np.float64), dev)
+        self.nab = to_device(alpha_beta(nel, spins), dev)
         self.n0 = to_device(n0.astype(np.float64), dev)
         self.kt = self.opts.etemp * param.KELVIN2AU
         self.nvec = ns + (9 * na if self.mp else 0)

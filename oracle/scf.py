@@ -51,7 +51,7 @@ from .basis import Basis, build
 
 class System:
     def __init__(self, numbers, positions: torch.Tensor, gfn: str, charge: float = 0.0, etemp: float = 300.0,
-                 solvation: dict | None = None):
+                 solvation: dict | None = None, spin: int | None = None):
         # solvation: generalized Born, {"eps": dielectric constant, "alpb": bool, "kernel": "p16" | "still"}
         self.solvation = solvation
         self.numbers = [int(z) for z in numbers if int(z) != 0]
@@ -74,10 +74,11 @@ class System:
         nsph = torch.tensor([2 * sh.l + 1 for sh in b.shells], dtype=torch.float64)
         self.n0 = (refocc / nsph)[self.ao_shell]
         nel = refocc.sum() - self.charge
-        # alpha/beta split for a closed/open shell without explicit spin
-        # (filling.py:180-259)
+        # alpha/beta split (filling.py:180-259): spin unpaired electrons if it
+        # has the parity of the electron count, else the parity alone
         nint = torch.floor(nel + 0.5)
-        nuhf = torch.remainder(nint, 2)
+        par = torch.remainder(nint, 2)
+        nuhf = par if spin is None or float(spin) % 2 != float(par) else torch.tensor(float(spin), dtype=torch.float64)
         diff = torch.minimum(nuhf, nel)
         nb = (nel - diff) / 2.0
         self.nab = torch.stack([nb + diff, nb])
@@ -322,12 +323,12 @@ class Result(dict):
 
 
 def singlepoint(numbers, positions, gfn, charge=0.0, tol=1e-10, maxiter=300, forces=False, solvation=None,
-                scf_mode="implicit", mixer="anderson", x_tol_max=1e-5):
+                scf_mode="implicit", mixer="anderson", x_tol_max=1e-5, spin=None):
     """Total energy (repulsion + electronic + -TS) and optionally forces;
     ``solvation`` adds generalized Born implicit solvation (see System).
     ``scf_mode="implicit"``: broyden1 on the residual; ``"full"``: the
     full-mode loop with ``mixer`` "anderson" or "simple" (x_tol = tol)."""
-    sysm = System(numbers, positions, gfn, charge, solvation=solvation)
+    sysm = System(numbers, positions, gfn, charge, solvation=solvation, spin=spin)
     with torch.no_grad():
         m = sysm.build(positions)
         n = sysm.nao + (9 * sysm.nat if sysm.is_gfn2 else 0)

@@ -31,9 +31,9 @@ def test_batched_padded(cuda, geometries):
     nums = torch.zeros((2, 5), dtype=torch.long)
     pos = torch.zeros((2, 5, 3), dtype=torch.float64)
     nums[0, :3] = torch.tensor(a["numbers"])
-    pos[0, :3] = torch.tensor(a["positions"])
+    pos[0, :3] = torch.tensor(a["positions"], dtype=torch.float64)
     nums[1, :5] = torch.tensor(b["numbers"])
-    pos[1, :5] = torch.tensor(b["positions"])
+    pos[1, :5] = torch.tensor(b["positions"], dtype=torch.float64)
     calc = GFN2Calculator(nums.to(cuda), device=cuda)
     e = calc.get_energy(pos.to(cuda), chrg=torch.tensor([0.0, 0.0]))
     for i, g in enumerate((a, b)):
@@ -117,3 +117,37 @@ def test_h2_bond_order_is_one(cuda, geometries):
     pos = torch.tensor(m["positions"], dtype=torch.float64, device=cuda)
     wbo = calc.get_bond_orders(pos).cpu()
     assert abs(wbo[0, 1].item() - 1.0) < 1e-8 and wbo[0, 0].item() == 0.0
+
+
+def test_spin_vs_oracle(cuda, geometries):
+    """Open shells through the calculators' spin argument: NO2 as a quartet
+    (GFN2), water as a triplet (GFN1), and a batch with one spin per
+    molecule, against the oracle with the same alpha/beta counts."""
+    cases = [("NO2", GFN2Calculator, "gfn2", 3), ("H2O", GFN1Calculator, "gfn1", 2)]
+    for name, cls, gfn, spin in cases:
+        m = geometries[name]
+        nums = torch.tensor(m["numbers"], device=cuda)
+        pos = torch.tensor(m["positions"], dtype=torch.float64, device=cuda, requires_grad=True)
+        calc = cls(nums, device=cuda)
+        e = calc.get_energy(pos, spin=spin)
+        f = calc.get_forces(pos, spin=spin)
+        ref = oscf.singlepoint(m["numbers"], torch.tensor(m["positions"], dtype=torch.float64), gfn, tol=1e-11,
+                               spin=spin, forces=True)
+        assert abs(e.item() - ref["energy"].item()) < 1e-8, name
+        torch.testing.assert_close(f.cpu(), ref["forces"], atol=1e-7, rtol=0)
+        # the spin changes the result (else the argument was dropped)
+        assert abs(calc.get_energy(pos).item() - e.item()) > 1e-4, name
+    with pytest.raises(ValueError, match="Odd"):
+        GFN2Calculator(torch.tensor(geometries["NO2"]["numbers"], device=cuda), device=cuda).get_energy(
+            torch.tensor(geometries["NO2"]["positions"], dtype=torch.float64, device=cuda), spin=2)
+    a, b = geometries["H2O"], geometries["CH4"]
+    nums = torch.zeros((2, 5), dtype=torch.long)
+    pos = torch.zeros((2, 5, 3), dtype=torch.float64)
+    nums[0, :3], pos[0, :3] = torch.tensor(a["numbers"]), torch.tensor(a["positions"], dtype=torch.float64)
+    nums[1, :5], pos[1, :5] = torch.tensor(b["numbers"]), torch.tensor(b["positions"], dtype=torch.float64)
+    calc = GFN2Calculator(nums.to(cuda), device=cuda)
+    e = calc.get_energy(pos.to(cuda), spin=torch.tensor([2, 0]))
+    for i, (g, sp) in enumerate(((a, 2), (b, 0))):
+        ref = oscf.singlepoint(g["numbers"], torch.tensor(g["positions"], dtype=torch.float64), "gfn2", tol=1e-11,
+                               spin=sp)
+        assert abs(e[i].item() - ref["energy"].item()) < 1e-8

@@ -1,0 +1,46 @@
+"""Alpha/beta electron counts with an explicit number of unpaired electrons
+(the calculators' ``spin`` argument; wavefunction/filling.py:180-259):
+na + nb = nel, na - nb = the unpaired electrons, and the reference's errors
+for counts it rejects."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from dxtb_amd.scf import alpha_beta
+
+
+@pytest.mark.parametrize("nel,uhf,expect", [
+    (17.0, None, (9.0, 8.0)),      # odd count: one unpaired electron
+    (17.0, 3, (10.0, 7.0)),        # quartet
+    (8.0, None, (4.0, 4.0)),
+    (8.0, 2, (5.0, 3.0)),          # triplet
+    (8.0, 0, (4.0, 4.0)),
+    (1.0, 1, (1.0, 0.0)),
+    # fractional counts: the parity of the rounded count fixes the unpaired
+    # electrons, the rest is split equally
+    (7.5, None, (3.75, 3.75)),     # rounds to 8: even
+    (7.4, None, (4.2, 3.2)),       # rounds to 7: one unpaired
+])
+def test_split(nel, uhf, expect):
+    ab = alpha_beta(np.array([nel]), None if uhf is None else np.array([uhf]))
+    np.testing.assert_allclose(ab[0], expect, rtol=0, atol=1e-15)
+    assert abs(ab[0].sum() - nel) < 1e-15
+
+
+def test_batch_and_scalar_spin():
+    nel = np.array([8.0, 17.0, 10.0])
+    ab = alpha_beta(nel, np.array([2, 1, 4]))
+    np.testing.assert_array_equal(ab.sum(1), nel)
+    np.testing.assert_array_equal(ab[:, 0] - ab[:, 1], [2, 1, 4])
+    np.testing.assert_array_equal(alpha_beta(nel[[0, 2]], 2), [[5.0, 3.0], [6.0, 4.0]])
+
+
+def test_rejected_spins():
+    with pytest.raises(ValueError, match="larger than"):
+        alpha_beta(np.array([2.0]), np.array([4]))
+    with pytest.raises(ValueError, match="Odd"):
+        alpha_beta(np.array([17.0]), np.array([2]))
+    with pytest.raises(RuntimeError, match="Shape mismatch"):
+        alpha_beta(np.array([8.0, 8.0]), np.array([0, 2, 2]))
