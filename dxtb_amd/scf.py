@@ -200,8 +200,13 @@ def alpha_beta(nel: np.ndarray, uhf=None) -> np.ndarray:
     if uhf is None:
         uhf = np.zeros_like(nel)
     else:
+        if not torch.is_tensor(uhf) and not np.isscalar(uhf):
+            # a sequence may leave single molecules unset (None): their parity decides
+            uhf = [np.nan if u is None else u for u in uhf]
         uhf = np.asarray(uhf.cpu() if torch.is_tensor(uhf) else uhf, dtype=np.float64)
         uhf = np.broadcast_to(uhf, nel.shape) if uhf.ndim == 0 else uhf
+        if uhf.shape == nel.shape:
+            uhf = np.where(np.isnan(uhf), np.remainder(nint, 2), uhf)
         if uhf.shape != nel.shape:
             raise RuntimeError(f"Shape mismatch for unpaired electrons ({uhf.shape}) and number of electrons "
                                f"({nel.shape}).")

@@ -55,6 +55,7 @@ class Request:
     iterations: int = 0
     converged: bool = False
     solvation: object = None  # GeneralizedBorn settings, or None (vacuum)
+    spin: int | None = None  # unpaired electrons (None: from the parity of the electron count)
 
     @property
     def latency(self) -> float:
@@ -197,8 +198,10 @@ class SlotPool:
             P[i, : len(r.numbers)] = r.positions
         bb = build_batch(N, self.gfn, self.dev, self.dims)
         chg = np.array([r.charge for r in reqs], dtype=np.float64)
+        spins = [r.spin for r in reqs]
         st = hscf.GeometryState(bb, _to_device(P, self.dev), charges=chg, opts=self.opts,
-                                solvation=[r.solvation for r in reqs])
+                                solvation=[r.solvation for r in reqs],
+                                spins=None if all(x is None for x in spins) else spins)
         rows64 = np.asarray(rows, dtype=np.int64)
         rt = _to_device(rows64, self.dev)
         self.state.scatter(rows64, st)
@@ -555,7 +558,8 @@ def plain_path(req: Request, gfn: str, device, opts: hscf.SCFOptions | None = No
     N = np.asarray(req.numbers, dtype=np.int64)[None]
     bb = bb_(N, gfn, dev)
     st = hscf.GeometryState(bb, torch.as_tensor(np.asarray(req.positions)[None], device=dev),
-                            charges=np.array([req.charge]), opts=opts, solvation=[req.solvation])
+                            charges=np.array([req.charge]), opts=opts, solvation=[req.solvation],
+                            spins=None if req.spin is None else [req.spin])
     res = hscf.run_scf(st)
     f = hscf.forces(res)
     req.energy = float(res.energy[0])

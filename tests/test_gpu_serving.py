@@ -191,3 +191,22 @@ def test_compact_moves_slots_exactly(cuda):
         assert m.converged and ref.converged
         assert abs(m.energy - ref.energy) < 1e-9, (m.rid, m.energy - ref.energy)
         np.testing.assert_allclose(m.forces, ref.forces, atol=1e-7, rtol=0)
+
+
+def test_spin_requests(cuda, geometries):
+    """Requests with and without a spin share a pool: the triplet water and
+    the quartet NO2 match the calculators' results for the same spin, the
+    closed-shell water its singlet."""
+    from dxtb_amd.calculators import GFN2Calculator
+
+    reqs = []
+    for i, (name, spin) in enumerate((("H2O", 2), ("H2O", None), ("NO2", 3))):
+        g = geometries[name]
+        reqs.append(sv.Request(i, np.array(g["numbers"]), np.array(g["positions"], dtype=np.float64), spin=spin))
+    print("case serving spins", flush=True)
+    out = {r.rid: r for r in sv.Engine("gfn2", cuda).run(reqs, realtime=False)}
+    for i, (name, spin) in enumerate((("H2O", 2), ("H2O", None), ("NO2", 3))):
+        g = geometries[name]
+        calc = GFN2Calculator(torch.tensor(g["numbers"], device=cuda), device=cuda)
+        e = calc.get_energy(torch.tensor(g["positions"], dtype=torch.float64, device=cuda), spin=spin)
+        assert out[i].converged and abs(out[i].energy - e.item()) < 1e-9, (name, spin)

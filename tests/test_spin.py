@@ -44,3 +44,11 @@ def test_rejected_spins():
         alpha_beta(np.array([17.0]), np.array([2]))
     with pytest.raises(RuntimeError, match="Shape mismatch"):
         alpha_beta(np.array([8.0, 8.0]), np.array([0, 2, 2]))
+
+
+def test_unset_entries_follow_parity():
+    """A per-molecule sequence may leave molecules unset (serving requests
+    without a spin): those take the parity rule."""
+    ab = alpha_beta(np.array([17.0, 8.0, 8.0]), [None, 2, None])
+    np.testing.assert_array_equal(ab, [[9.0, 8.0], [5.0, 3.0], [4.0, 4.0]])
+    np.testing.assert_array_equal(alpha_beta(np.array([17.0, 8.0]), [None, None]), alpha_beta(np.array([17.0, 8.0])))
