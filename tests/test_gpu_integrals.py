@@ -1,7 +1,7 @@
 """
 HIP multipole integrals (S, D, Q) and their VJP against the oracle.
 
-Tolerance: 1e-11 absolute on every integral element (the north-star bar for
+Tolerance: 1e-11 absolute on every integral element (the accuracy bar for
 integrals); the VJP is compared with oracle autograd at 1e-10 absolute.
 """
 

@@ -1,5 +1,5 @@
 """
-GPU SCF energies against the oracle (|dE| < 1e-8 Eh),
+GPU SCF energies against the oracle (|dE| < 1e-8 Eh, the accuracy bar of the path),
 batched and single, GFN1 and GFN2, and against the reference's tblite GFN1
 SCF energies at the reference's own tolerance.
 """
