@@ -14,8 +14,12 @@ Mirrors the reference calculator API (``src/dxtb/_src/calculators/gfn1.py:38-70`
 * ``get_forces(positions, chrg=0, spin=None, grad_mode="autograd")`` -> -dE/dR;
   like the reference, it requires ``positions.requires_grad`` (RuntimeError
   otherwise); the forces come from the hand-written backward kernels;
-* results are cached by tensor identity of the inputs; ``reset()`` clears
-  the cache (``types/base.py:758-763``);
+* ``get_energy`` is differentiable in the positions (when they require
+  gradients): ``torch.autograd.grad(E, positions)`` runs the hand-written
+  force kernels (``types/autograd.py:146-150`` differentiates the same way);
+* the last result is cached (one entry, as ``types/decorators.py:241-289``)
+  and reused for inputs of equal value; ``reset()`` clears the cache
+  (``types/base.py:758-763``);
 * the single-point properties of the reference's energy calculator
   (``types/energy.py:55-65``): ``get_iterations``, ``get_density``,
   ``get_mo_energies`` (ascending), ``get_occupation``, ``get_coefficients``,
@@ -29,7 +33,9 @@ Mirrors the reference calculator API (``src/dxtb/_src/calculators/gfn1.py:38-70`
   "broyden" falls back to Anderson with a warning, as in
   ``scf/unrolling/base.py:81-98``), ``maxiter``, ``x_atol``, ``f_atol``,
   ``x_atol_max``, ``damp``, ``damp_init``, ``damp_generations``,
-  ``damp_soft_start``, ``fermi_etemp``.
+  ``damp_soft_start``, ``fermi_etemp``; ``scp_mode``, ``int_driver`` and
+  ``exclude`` warn where they differ from what this path does, or raise
+  where it cannot follow them.
 
 Dispersion (D3/D4) is not part of this path: the energy is the reference's
 total energy with ``exclude=["disp"]`` (``types/base.py:612-663``).
@@ -92,7 +98,46 @@ def _scf_mode(val) -> str:
     raise ValueError(f"Unknown SCF mode '{val}'. Use one of 'implicit', 'full'.")
 
 
-def _opts_to_scf(opts: dict | None) -> _scf.SCFOptions:
+# components of the reference calculator (calculators/types/base.py:596-671)
+# and what this path computes: ES2, ES3, AES2 (GFN2) and repulsion always;
+# dispersion (D3/D4, D4SC) never; the GFN1 halogen-bond correction, the
+# isotropic electrostatics and short-range basis terms (not part of GFN1/GFN2
+# single points with these parametrizations' defaults) never
+_EXCLUDE_ALWAYS = {"disp", "d4sc", "hal", "ies", "srb"}
+_EXCLUDE_NEVER = {"es2", "es3", "aes2", "rep", "all"}
+# halogen-bond donors of the GFN1 correction (classicals/halogen.py)
+_HALOGEN_DONORS = (17, 35, 53, 85)
+
+
+def _check_exclude(val, gfn: str, numbers: np.ndarray) -> None:
+    """The reference's ``exclude`` list (``calculators/types/base.py:596-671``):
+    components this path cannot switch off raise NotImplementedError; a
+    list that keeps a component this path never computes warns."""
+    exc = {str(v).casefold() for v in ([val] if isinstance(val, str) else list(val))}
+    bad = exc & _EXCLUDE_NEVER
+    if bad:
+        raise NotImplementedError(f"Excluding {sorted(bad)} is not supported on the MI355X path; it always computes "
+                                  "ES2, ES3, AES2 (GFN2) and repulsion.")
+    unknown = exc - _EXCLUDE_ALWAYS
+    if unknown:
+        raise ValueError(f"Unknown component(s) {sorted(unknown)} in 'exclude'.")
+    if "disp" not in exc:
+        warnings.warn("Dispersion (D3/D4, D4SC) is not computed on the MI355X path: the energy is the reference's "
+                      "energy with exclude=['disp'].", UserWarning, stacklevel=4)
+    if gfn == "gfn1" and "hal" not in exc and np.isin(numbers, _HALOGEN_DONORS).any():
+        warnings.warn("The GFN1 halogen-bond correction is not computed on the MI355X path: the energy is the "
+                      "reference's energy with 'hal' excluded.", UserWarning, stacklevel=4)
+
+
+def _opts_to_scf(opts: dict | None, gfn: str = "gfn2", numbers: np.ndarray | None = None) -> _scf.SCFOptions:
+    """The reference's options (``calculators/config/*.py``) as SCF options
+    of this path.  Accepted and used: the SCF tolerances, ``maxiter``,
+    ``fermi_etemp``, the mixer settings, ``scf_mode``, ``mixer``; accepted
+    with a warning where the value differs from what this path does:
+    ``scp_mode`` (the potential is iterated: the same fixed point as
+    "charge"/"fock"), ``int_driver`` (the HIP integral driver: the same
+    integrals as "libcint"), ``exclude`` (see _check_exclude); accepted:
+    ``verbosity``, ``batch_mode``, ``force_convergence``."""
     o = _scf.SCFOptions()
     if not opts:
         return o
@@ -123,7 +168,25 @@ def _opts_to_scf(opts: dict | None) -> _scf.SCFOptions:
             if m is None:
                 raise ValueError(f"Unknown mixer '{val}'.")
             mixer = m
-        elif key in ("verbosity", "scp_mode", "int_driver", "batch_mode", "force_convergence", "exclude"):
+        elif key == "scp_mode":
+            # constants/labels/scf.py:144-160: fock 0, charge 1, potential 2
+            v = val.casefold() if isinstance(val, str) else val
+            if v not in ("potential", "pot", 2):
+                if v not in ("charge", "charges", "fock", "fockian", 0, 1):
+                    raise ValueError(f"Unknown scp_mode '{val}'.")
+                warnings.warn(f"scp_mode '{val}': the MI355X path iterates the potential (the same fixed point).",
+                              UserWarning, stacklevel=3)
+        elif key == "int_driver":
+            # constants/labels/integrals.py:45-54: libcint 0, pytorch 1
+            v = val.casefold() if isinstance(val, str) else val
+            if v not in ("dxtb", "torch", "pytorch", "autograd", "hip", 1):
+                if v not in ("libcint", "c", 0):
+                    raise ValueError(f"Unknown integral driver '{val}'.")
+                warnings.warn(f"int_driver '{val}': the MI355X path computes the integrals with its HIP driver "
+                              "(the same integrals).", UserWarning, stacklevel=3)
+        elif key == "exclude":
+            _check_exclude(val, gfn, np.zeros(0) if numbers is None else numbers)
+        elif key in ("verbosity", "batch_mode", "force_convergence"):
             continue
         else:
             raise ValueError(f"Unknown option '{key}'.")
@@ -155,7 +218,7 @@ class _Calculator:
         self.batched = numbers.ndim == 2
         nums = np.atleast_2d(numbers.detach().cpu().numpy())
         self.opts = dict(opts or {})
-        self.scf_opts = _opts_to_scf(opts)
+        self.scf_opts = _opts_to_scf(opts, self.gfn, nums)
         self.force_convergence = bool(self.opts.get("force_convergence", False))
         self.bb = build_batch(nums, self.gfn, self.device)
         # extra interactions (calculators/types/base.py: `interaction` list):
@@ -197,16 +260,33 @@ class _Calculator:
         s = np.asarray(spin.detach().cpu() if torch.is_tensor(spin) else spin, dtype=np.float64)
         return np.broadcast_to(s, (B,)).copy() if s.ndim == 0 else s.reshape(B)
 
-    def _key(self, positions, chrg, spin):
-        cid = id(chrg) if torch.is_tensor(chrg) else chrg
-        sid = id(spin) if torch.is_tensor(spin) else spin
-        return (id(positions), positions._version, cid, sid)
+    @staticmethod
+    def _value_key(v):
+        """Hashable value of a charge or spin argument (scalar, sequence or tensor)."""
+        if v is None:
+            return None
+        a = np.asarray(v.detach().cpu() if torch.is_tensor(v) else v, dtype=object)
+        return tuple(None if x is None else float(x) for x in a.reshape(-1))
+
+    def _cached(self, positions, chrg, spin):
+        """The cached result if it was computed for the same inputs: one entry,
+        as the reference's cache (``types/decorators.py:241-289``), validated by
+        the values of the positions, charges and spins (tensor identity alone
+        would match a new tensor that reuses a freed tensor's id)."""
+        c = self.cache.get("singlepoint")
+        if c is None:
+            return None
+        pos, chrg_key, spin_key, res = c
+        if (chrg_key != self._value_key(chrg) or spin_key != self._value_key(spin) or pos.shape != positions.shape
+                or not torch.equal(pos, positions.detach())):
+            return None
+        return res
 
     def singlepoint(self, positions, chrg=0, spin=None):
         self._check(positions)
-        key = self._key(positions, chrg, spin)
-        if key in self.cache:
-            return self.cache[key]
+        res = self._cached(positions, chrg, spin)
+        if res is not None:
+            return res
         pos = positions.detach().reshape(self.bb.nbatch, self.bb.nat_max, 3)
         state = _scf.GeometryState(self.bb, pos, self._charges(chrg), self.scf_opts, solvation=self.solvation,
                                    spins=self._spins(spin))
@@ -216,12 +296,20 @@ class _Calculator:
             if self.force_convergence:
                 raise SCFConvergenceError(msg)
             warnings.warn(msg, SCFConvergenceWarning, stacklevel=3)
-        self.cache[key] = res
+        self.cache["singlepoint"] = (positions.detach().clone(), self._value_key(chrg), self._value_key(spin), res)
         self._ncalcs += 1
         return res
 
     def get_energy(self, positions, chrg=0, spin=None, **_):
-        e = self.singlepoint(positions, chrg, spin).energy
+        """Total energy.  With ``positions.requires_grad`` (and grad mode on)
+        the energy is differentiable in the positions: its backward runs the
+        hand-written force kernels, so ``torch.autograd.grad(E, positions)``
+        is the gradient the reference gets from autograd
+        (``calculators/types/autograd.py:146-150``).  First derivatives only."""
+        if positions.requires_grad and torch.is_grad_enabled():
+            e = _EnergyFunction.apply(positions, self, chrg, spin)
+        else:
+            e = self.singlepoint(positions, chrg, spin).energy
         return e if self.batched else e[0]
 
     def get_forces(self, positions, chrg=0, spin=None, grad_mode="autograd", **_):
@@ -310,6 +398,28 @@ class _Calculator:
         if name not in getters:
             raise NotImplementedError(f"Property '{name}' is not implemented on the MI355X path.")
         return getters[name](positions, chrg, spin, **kw)
+
+
+class _EnergyFunction(torch.autograd.Function):
+    """SCF total energy as a function of the positions.  Backward: the
+    gradient of the converged free energy from the hand-written kernels
+    (integral derivatives contracted with P and W, H0/CN, electrostatics,
+    repulsion and solvation terms; ``scf.forces``), times the incoming
+    gradient of every molecule."""
+
+    @staticmethod
+    def forward(ctx, positions, calc, chrg, spin):
+        res = calc.singlepoint(positions, chrg, spin)
+        ctx.res = res
+        ctx.shape = positions.shape
+        return res.energy.clone()
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, grad_e):
+        g = -_scf.forces(ctx.res)  # (B, nat_max, 3) = dE/dR
+        g = g * grad_e.reshape(-1, 1, 1).to(g.dtype)
+        return g.reshape(ctx.shape), None, None, None
 
 
 class GFN1Calculator(_Calculator):

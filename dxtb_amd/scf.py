@@ -231,11 +231,17 @@ class GeometryState:
         self.gfn = bb.gfn
         self.mp = bb.gfn == "gfn2"
         dev = positions.device
+        if dev.type != "cuda":
+            raise RuntimeError(f"positions must be on a GPU device, not {dev} (there is no CPU path)")
+        if positions.numel() != bb.nbatch * bb.nat_max * 3:
+            raise ValueError(f"positions hold {positions.numel()} values, the basis needs "
+                             f"{bb.nbatch} x {bb.nat_max} x 3")
         self.dev = dev
         self.lib = _lib.load()
         self.dp = device_params(bb.gfn, dev)
         self.bs = _lib.basis_struct(bb)
-        self.pos = positions.detach().reshape(bb.nbatch, bb.nat_max, 3).contiguous()
+        # the kernels read FP64 coordinates
+        self.pos = positions.detach().to(torch.float64).reshape(bb.nbatch, bb.nat_max, 3).contiguous()
         d = bb.dev
         B, na, ns, n = bb.nbatch, bb.nat_max, bb.nsh_max, bb.nao_max
         f64 = dict(dtype=torch.float64, device=dev)
@@ -805,6 +811,7 @@ def run_scf(state: GeometryState, cull: float = 0.75) -> SCFResult:
             mix.C, mix.D, mix.w = mix.C[keep], mix.D[keep], cur.wvec
     if bool(active.any()):
         x_full[idx[active]] = x[active]
+    state._jtol = None  # the final solve of every molecule at the tight bar
     return finalize(state, x_full, iters, conv)
 
 

@@ -274,6 +274,9 @@ class SlotPool:
         win = self._wins.get(W)
         if win is None:
             win = self._wins[W] = self.state.window(W)
+        # the solvation flags of the pool change when a GBSA request is admitted
+        # after the window was made (its tensors are views of the pool's rows)
+        win.any_gb, win._alpb, win.gb_par = self.state.any_gb, self.state._alpb, self.state.gb_par
         occ, started = self.d_ctl[0, :W].bool(), self.d_ctl[1, :W].bool()
         kslot = self.d_ctl[2, :W].clamp(max=self.hc.shape[1] - 1)
         x, y = self.x[:W], self.y[:W]
@@ -557,7 +560,7 @@ def plain_path(req: Request, gfn: str, device, opts: hscf.SCFOptions | None = No
     dev = torch.device(device)
     N = np.asarray(req.numbers, dtype=np.int64)[None]
     bb = bb_(N, gfn, dev)
-    st = hscf.GeometryState(bb, torch.as_tensor(np.asarray(req.positions)[None], device=dev),
+    st = hscf.GeometryState(bb, torch.as_tensor(np.asarray(req.positions, dtype=np.float64)[None], device=dev),
                             charges=np.array([req.charge]), opts=opts, solvation=[req.solvation],
                             spins=None if req.spin is None else [req.spin])
     res = hscf.run_scf(st)

@@ -151,3 +151,60 @@ def test_spin_vs_oracle(cuda, geometries):
         ref = oscf.singlepoint(g["numbers"], torch.tensor(g["positions"], dtype=torch.float64), "gfn2", tol=1e-11,
                                spin=sp)
         assert abs(e[i].item() - ref["energy"].item()) < 1e-8
+
+
+@pytest.mark.parametrize("cls,gfn", [(GFN1Calculator, "gfn1"), (GFN2Calculator, "gfn2")])
+def test_energy_is_differentiable(cuda, geometries, cls, gfn):
+    """torch.autograd.grad of get_energy in the positions (the reference's
+    forces, calculators/types/autograd.py:146-150) runs the hand-written
+    force kernels: it equals -get_forces and the oracle's forces."""
+    m = geometries["MB16_43_01"]
+    print(f"case autograd energy {gfn}", flush=True)
+    nums = torch.tensor(m["numbers"], device=cuda)
+    pos = torch.tensor(m["positions"], dtype=torch.float64, device=cuda, requires_grad=True)
+    calc = cls(nums, device=cuda)
+    e = calc.get_energy(pos)
+    assert e.requires_grad
+    (g,) = torch.autograd.grad(e, pos)
+    f = calc.get_forces(pos)
+    torch.testing.assert_close(g, -f, atol=1e-12, rtol=0)
+    ref = oscf.singlepoint(m["numbers"], torch.tensor(m["positions"], dtype=torch.float64), gfn, tol=1e-11,
+                           forces=True)
+    torch.testing.assert_close(-g.cpu(), ref["forces"], atol=1e-7, rtol=0)
+    # a weighted batch: each molecule's gradient scaled by its incoming gradient
+    a, b = geometries["H2O"], geometries["CH4"]
+    nb = torch.zeros((2, 5), dtype=torch.long)
+    pb = torch.zeros((2, 5, 3), dtype=torch.float64)
+    nb[0, :3], pb[0, :3] = torch.tensor(a["numbers"]), torch.tensor(a["positions"], dtype=torch.float64)
+    nb[1, :5], pb[1, :5] = torch.tensor(b["numbers"]), torch.tensor(b["positions"], dtype=torch.float64)
+    pb = pb.to(cuda).requires_grad_(True)
+    cb = cls(nb.to(cuda), device=cuda)
+    eb = cb.get_energy(pb, chrg=torch.tensor([0.0, 0.0]))
+    (gb,) = torch.autograd.grad((eb * torch.tensor([2.0, -3.0], device=cuda, dtype=torch.float64)).sum(), pb)
+    fb = cb.get_forces(pb, chrg=torch.tensor([0.0, 0.0]))
+    torch.testing.assert_close(gb[0], -2.0 * fb[0], atol=1e-12, rtol=0)
+    torch.testing.assert_close(gb[1], 3.0 * fb[1], atol=1e-12, rtol=0)
+
+
+def test_cache_follows_values(cuda, geometries):
+    """The result cache holds one entry and is validated by the input values:
+    a loop ``pos = pos + d`` (new tensors that may reuse freed ids) gets a new
+    energy at every step; equal values reuse the result; list charges work."""
+    m = geometries["H2O"]
+    calc = GFN2Calculator(torch.tensor(m["numbers"], device=cuda), device=cuda)
+    pos = torch.tensor(m["positions"], dtype=torch.float64, device=cuda)
+    d = torch.zeros_like(pos)
+    d[0, 2] = 1e-3
+    energies = []
+    for _ in range(6):
+        pos = pos + d
+        energies.append(calc.get_energy(pos).item())
+    assert all(abs(x - y) > 1e-9 for x, y in zip(energies, energies[1:]))
+    assert len(calc.cache) == 1
+    ncalc = calc._ncalcs
+    calc.get_energy(pos.clone())  # same values: cached
+    assert calc._ncalcs == ncalc
+    e_plus = calc.get_energy(pos, chrg=[1.0])
+    assert calc._ncalcs == ncalc + 1 and abs(e_plus.item() - energies[-1]) > 1e-3
+    ref = oscf.singlepoint(m["numbers"], pos.cpu(), "gfn2", tol=1e-11)
+    assert abs(calc.get_energy(pos).item() - ref["energy"].item()) < 1e-8

@@ -210,3 +210,58 @@ def test_spin_requests(cuda, geometries):
         calc = GFN2Calculator(torch.tensor(g["numbers"], device=cuda), device=cuda)
         e = calc.get_energy(torch.tensor(g["positions"], dtype=torch.float64, device=cuda), spin=spin)
         assert out[i].converged and abs(out[i].energy - e.item()) < 1e-9, (name, spin)
+
+
+def test_gbsa_admitted_into_vacuum_pool(cuda):
+    """A pool iterates vacuum molecules over a window of two slots; when the
+    small one (H2) has retired, a GBSA request is admitted into its slot
+    while the other is mid-SCF, so the pool keeps iterating the same window.
+    The GBSA molecule is iterated with its solvation term (the window follows
+    the pool's solvation flags): its energy and forces equal the
+    one-molecule path's, and so do the vacuum molecule's."""
+    from dxtb_amd.solvation import GeneralizedBorn
+
+    nums, poss, chg = workloads.config_batch(3, count=200)
+    c48 = [i for i in range(len(nums)) if sv.pool_class(*sv.sizes(nums[i], "gfn2")) == 48 and len(nums[i]) >= 8]
+    assert len(c48) >= 2
+    gb = GeneralizedBorn(None, 78.4, alpb=False)
+    h2 = sv.Request(-1, np.array([1, 1]), np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.4]]))
+    slow = sv.Request(c48[0], nums[c48[0]], poss[c48[0]], float(chg[c48[0]]))
+    late = sv.Request(c48[1], nums[c48[1]], poss[c48[1]], float(chg[c48[1]]), solvation=gb)
+    print("case gbsa admission into a vacuum pool", c48[:2], flush=True)
+    eng = sv.Engine("gfn2", cuda, capacity={48: 4})
+    eng.submit(h2)
+    eng.submit(slow)
+    out = []
+    while not any(r.rid == -1 for r in out):
+        out += eng.tick()
+    p = eng.pools[48]
+    assert p.occupied.tolist() == [False, True, False, False] and not p.state.any_gb
+    assert 2 in p._wins  # the two-slot window was made while the pool was in vacuum
+    eng.submit(late)
+    out += eng.tick()
+    assert p.occupied[0] and p.req[0] is late and p.hi == 2 and p.admitted_busy == 1
+    while eng.busy:
+        out += eng.tick()
+    got = {r.rid: r for r in out}
+    for r in (slow, late):
+        ref = sv.plain_path(sv.Request(r.rid, r.numbers, r.positions, r.charge, solvation=r.solvation), "gfn2",
+                            cuda)
+        m = got[r.rid]
+        print(f"case {r.rid} gb={r.solvation is not None}: dE {m.energy - ref.energy:.2e}", flush=True)
+        assert m.converged and ref.converged
+        assert abs(m.energy - ref.energy) < 1e-9
+        np.testing.assert_allclose(m.forces, ref.forces, atol=1e-7, rtol=0)
+
+
+def test_float32_request_positions(cuda):
+    """Positions given in float32 are converted to FP64 before any kernel
+    reads them (the plain path and the engine agree with the FP64 request)."""
+    r = _requests(k=1, maxnat=16)[0]
+    r32 = sv.Request(r.rid, r.numbers, np.asarray(r.positions, dtype=np.float32), r.charge)
+    r64 = sv.Request(r.rid, r.numbers, np.asarray(r32.positions, dtype=np.float64), r.charge)
+    a = sv.plain_path(r32, "gfn2", cuda)
+    b = sv.plain_path(r64, "gfn2", cuda)
+    assert abs(a.energy - b.energy) < 1e-12
+    c = sv.Engine("gfn2", cuda).run([sv.Request(r.rid, r.numbers, r32.positions, r.charge)], realtime=False)[0]
+    assert abs(c.energy - b.energy) < 1e-9
