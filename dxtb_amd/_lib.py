@@ -96,6 +96,8 @@ _SIGNATURES: dict[str, list] = {
     "dxtb_jacobi_eigh": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                          c_double, c_void_p, c_void_p, c_void_p],
     "dxtb_forces_classical": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 19,
+    "dxtb_gemm_batched": [c_int, c_int, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_double,
+                          c_void_p, c_void_p, c_int, c_int, c_double, c_void_p],
 }
 
 _lib = None
@@ -164,3 +166,28 @@ def basis_struct(bb) -> DxtbBasis:
     for i in range(10):
         s.class_off[i] = int(bb.class_off[i])
     return s
+
+
+def gemm(A: torch.Tensor, B: torch.Tensor, *, transa: bool = False, transb: bool = False, alpha: float = 1.0,
+         scale: torch.Tensor | None = None, beta: float = 0.0, cin: torch.Tensor | None = None,
+         out: torch.Tensor | None = None, nsize: torch.Tensor | None = None, sym: bool = False,
+         pad: float | None = None) -> torch.Tensor:
+    """Batched FP64 product alpha op(A) diag(scale) op(B) + beta cin on the
+    matrix cores (dxtb_gemm_batched) for (nbatch, n, n) tensors; ``nsize``
+    (int32, device) limits each product to its molecule's leading block;
+    ``pad`` (None: leave) fills the rest with pad * identity."""
+    lib = load()
+    if A.dim() != 3 or A.shape != B.shape or A.shape[1] != A.shape[2] or A.dtype != torch.float64 \
+            or B.dtype != torch.float64:
+        raise ValueError("gemm takes two FP64 (nbatch, n, n) tensors of one shape")
+    if out is None:
+        out = torch.empty_like(A)
+    if scale is not None and (scale.shape != A.shape[:2] or scale.dtype != torch.float64):
+        raise ValueError("scale must be an FP64 (nbatch, n) tensor")
+    if nsize is not None and (nsize.dtype != torch.int32 or nsize.numel() < A.shape[0]):
+        raise ValueError("nsize must be an int32 tensor with one entry per matrix")
+    rc = lib.dxtb_gemm_batched(A.shape[0], A.shape[1], ptr(nsize), int(transa), int(transb), float(alpha), ptr(A),
+                               ptr(B), ptr(scale), float(beta), ptr(cin), ptr(out), int(sym), int(pad is not None),
+                               float(pad or 0.0), stream_ptr(A.device))
+    check(rc, "dxtb_gemm_batched")
+    return out

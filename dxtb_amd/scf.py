@@ -299,6 +299,9 @@ class GeometryState:
         # after 170 iterations of a 628-function molecule: 2e-8 surplus
         # electrons, 1e-8 Eh), so U is re-orthogonalised every few solves
         self.U = None
+        # the previous iteration's orbitals C = X U (None: X), kept so that
+        # the next solve starts from them without recomputing X U
+        self.Corb = None
         self._jwork = None  # Jacobi workspace (a subset keeps the larger buffer)
         self._jtol = None  # per-molecule Jacobi tolerance of the next solve (None: JACOBI_TOL)
         self._jhist = _SolveHistory(bb.nao_max)  # shared by windows and subsets of this state
@@ -336,7 +339,7 @@ class GeometryState:
 
     # ---- culling: a state restricted to some molecules of the batch --------
     _PER_MOLECULE = ("pos", "cn", "fsh", "dfsh", "hfsh", "erep", "mrad", "S", "D", "Q", "H0", "X", "nab", "n0",
-                     "valid_sh", "U", "gb_keps", "gb_kind", "gb_alpbet", "gb_adet", "born", "dborn", "wvec")
+                     "valid_sh", "U", "Corb", "gb_keps", "gb_kind", "gb_alpbet", "gb_adet", "born", "dborn", "wvec")
     _GLOBAL_DEV = ("ush_l", "ush_np", "ush_alpha", "ush_coeff", "item_mol", "item_pair")
 
     _BB_HOST = ("numbers", "nat", "nsh", "nao", "sh_atom", "sh_ush", "sh_ao", "sh_l", "ao_atom", "ao_shell")
@@ -411,7 +414,7 @@ class GeometryState:
             rc = np.asarray(rows, dtype=np.int64)
             rows = to_device(rc, self.dev)
         for name in self._PER_MOLECULE:
-            if name == "U":
+            if name in ("U", "Corb"):
                 continue
             t = getattr(self, name)
             if t is not None:
@@ -420,6 +423,9 @@ class GeometryState:
         if self.U is None:
             self.U = torch.eye(n, dtype=torch.float64, device=self.dev).expand(ob.nbatch, n, n).contiguous()
         self.U.index_copy_(0, rows, torch.eye(n, dtype=torch.float64, device=self.dev).expand(len(rc), n, n))
+        if self.Corb is None:
+            self.Corb = self.X.clone()
+        self.Corb.index_copy_(0, rows, other.X)
         self._jhist.fresh()
         self.any_gb = self.any_gb or other.any_gb
         self._alpb = self._alpb or other._alpb
@@ -484,12 +490,16 @@ class GeometryState:
         C = X U the previous iteration's orbitals (S-orthonormal, so A is the
         Fock matrix in the old orbital basis and nearly diagonal; the first
         iteration uses U = 1, C = X = L^-T); the new orbitals are C V and U
-        becomes U V.  Larger bases (and eigensolver="device"): X^T F X and
-        the device eigh (rocSOLVER)."""
+        becomes U V.  The products run on the matrix cores
+        (dxtb_gemm_batched), each over its molecule's own basis size.  Larger
+        bases (and eigensolver="device"): X^T F X and the device eigh
+        (rocSOLVER)."""
         n = self.bb.nao_max
+        nsz = self.bb.dev["nao"]
+        gemm = _lib.gemm
         if self.opts.eigensolver == "jacobi" and n <= self.lib.dxtb_jacobi_max_n():
-            Cb = self.X if self.U is None else torch.bmm(self.X, self.U)
-            A = (Cb.transpose(-1, -2) @ F @ Cb).contiguous()
+            Cb = self.X if self.Corb is None else self.Corb
+            A = gemm(Cb, gemm(F, Cb, nsize=nsz), transa=True, nsize=nsz, sym=True)
             V = torch.empty_like(A)
             e = torch.empty((self.bb.nbatch, n), dtype=torch.float64, device=self.dev)
             nbytes = self.lib.dxtb_jacobi_work_size(self.bb.nbatch, n)
@@ -508,7 +518,7 @@ class GeometryState:
             if prof is not None:
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 ev[0].record(torch.cuda.current_stream(self.dev))
-            rc = self.lib.dxtb_jacobi_eigh(nb, n, _lib.ptr(self.bb.dev["nao"]), _lib.ptr(A), _lib.ptr(V),
+            rc = self.lib.dxtb_jacobi_eigh(nb, n, _lib.ptr(nsz), _lib.ptr(A), _lib.ptr(V),
                                            _lib.ptr(e), _lib.ptr(self._jwork), variant, 50, split, JACOBI_TOL,
                                            _lib.ptr(self._jtol), _lib.ptr(sw), _lib.stream_ptr(self.dev))
             _lib.check(rc, "dxtb_jacobi_eigh")
@@ -516,17 +526,22 @@ class GeometryState:
             if prof is not None:
                 ev[1].record(torch.cuda.current_stream(self.dev))
                 prof.append((ev[0], ev[1], n, np.array(self.bb.nao), sw.clone(), variant, split))
-            Vt = V.transpose(1, 2).contiguous()  # V holds the eigenvectors as rows
-            C = torch.bmm(Cb, Vt)
-            Un = Vt if self.U is None else torch.bmm(self.U, Vt)
+            # V holds the eigenvectors as rows: C V^T and U V^T
+            C = gemm(Cb, V, transb=True, nsize=nsz, pad=1.0)
+            Un = V.transpose(1, 2).contiguous() if self.U is None else gemm(self.U, V, transb=True, nsize=nsz,
+                                                                             pad=1.0)
             if self._jhist.reorthogonalise():
-                # one Newton-Schulz step, U (3 - U^T U) / 2: U^T U = 1 + E becomes 1 + O(E^2)
-                G = torch.bmm(Un.transpose(1, 2).contiguous(), Un)
-                Un = 1.5 * Un - 0.5 * torch.bmm(Un, G)
+                # one Newton-Schulz step, U (3 - U^T U) / 2: U^T U = 1 + E becomes 1 + O(E^2);
+                # the orbitals are then rebuilt from the re-orthogonalised U
+                G = gemm(Un, Un, transa=True, nsize=nsz, sym=True, pad=1.0)
+                Un = gemm(Un, G, alpha=-0.5, beta=1.5, cin=Un, nsize=nsz, pad=1.0)
+                C = gemm(self.X, Un, nsize=nsz, pad=1.0)
             if self.U is None:
                 self.U = Un
+                self.Corb = C
             else:
                 self.U.copy_(Un)  # in place: a window writes through to its parent state
+                self.Corb.copy_(C)
         else:
             Fp = self.X.transpose(-1, -2) @ F @ self.X
             side = torch.cuda.current_stream(self.dev) != torch.cuda.default_stream(self.dev)
@@ -535,17 +550,17 @@ class GeometryState:
             e, Cp = torch.linalg.eigh(Fp)
             if side:
                 torch.cuda.synchronize(self.dev)
-            C = self.X @ Cp
+            C = gemm(self.X, Cp.contiguous(), nsize=nsz, pad=1.0)
         occ = torch.empty((self.bb.nbatch, 2, self.bb.nao_max), dtype=torch.float64, device=self.dev)
         ts = torch.empty(self.bb.nbatch, dtype=torch.float64, device=self.dev)
         e = e.contiguous()
-        rc = self.lib.dxtb_fermi(self.bb.nbatch, self.bb.nao_max, _lib.ptr(self.bb.dev["nao"]), _lib.ptr(e),
+        rc = self.lib.dxtb_fermi(self.bb.nbatch, self.bb.nao_max, _lib.ptr(nsz), _lib.ptr(e),
                                  _lib.ptr(self.nab), self.kt, _lib.ptr(occ), _lib.ptr(ts),
                                  _lib.stream_ptr(self.dev))
         _lib.check(rc, "dxtb_fermi")
         f = occ.sum(1)
-        P = (C * f.unsqueeze(-2)) @ C.transpose(-1, -2)
-        return P.contiguous(), e, C, occ, ts
+        P = gemm(C, C, transb=True, scale=f, nsize=nsz, sym=True, pad=0.0)
+        return P, e, C, occ, ts
 
     def charges(self, P):
         bb, d = self.bb, self.bb.dev
@@ -885,7 +900,8 @@ def forces(res: SCFResult) -> torch.Tensor:
     bb, d = st.bb, st.bb.dev
     lib = st.lib
     f = res.occupation.sum(1)
-    W = ((res.coeffs * (f * res.emo).unsqueeze(-2)) @ res.coeffs.transpose(-1, -2)).contiguous()
+    C = res.coeffs.contiguous()
+    W = _lib.gemm(C, C, transb=True, scale=(f * res.emo).contiguous(), nsize=d["nao"], sym=True, pad=0.0)
     grad = torch.zeros((bb.nbatch, bb.nat_max, 3), dtype=torch.float64, device=st.dev)
     vtot, vdp, vqp = res.potential
     vtot = vtot.contiguous()

@@ -133,6 +133,7 @@ class SlotPool:
         self.state = hscf.GeometryState(bb, torch.as_tensor(pos, device=self.dev), opts=opts)
         self.state.U = torch.eye(nao_max, dtype=torch.float64, device=self.dev).expand(capacity, nao_max,
                                                                                       nao_max).contiguous()
+        self.state.Corb = self.state.X.clone()
         nbytes = self.state.lib.dxtb_jacobi_work_size(capacity, nao_max)
         self.state._jwork = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=self.dev)  # shared by windows
         L = self.state.nvec

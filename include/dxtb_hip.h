@@ -207,6 +207,26 @@ int dxtb_gb_forces(int nbatch, int nat_max, const int* nat, const int* numbers, 
                    const int* kind, const double* rvdw_z, const dxtb_gb_params_t* par, const double* born,
                    const double* dborn, const double* qat, double* grad, hipStream_t stream);
 
+/* Batched FP64 product on the matrix cores, for every matrix b of a batch of
+ * padded (n, n) row-major matrices:
+ *   Out_b = alpha * op(A_b) diag(scale_b) op(B_b) + beta * Cin_b
+ * op(X) = X or X^T (transa/transb != 0).  Only the leading (m, m) block,
+ * m = nsize[b] (n if nsize is NULL), is computed, with the sum over k running
+ * to m: the operands must be block diagonal with respect to the padding (the
+ * SCF's matrices are).  Outside that block Out gets pad on the diagonal and
+ * 0 elsewhere when fill_pad != 0, else it is left as it is.  sym != 0: the
+ * product is symmetric; only the tiles on and above the diagonal are computed
+ * and written to both triangles (Cin must then be symmetric too).  scale
+ * (nbatch, n) may be NULL; Cin may be NULL when beta == 0 and may alias Out;
+ * Out must not alias A or B.
+ * Replaces the dense products of the SCF step: the reference's
+ * L^-1 F L^-T and L^-T C of xitorch exacteig (exlibs/xitorch/_impls/linalg/
+ * symeig.py:38-75 via scf/pure/ovlp_diag.py:36-62) in the previous orbital
+ * basis, and the densities of scf/utils.py:29-52 (get_density). */
+int dxtb_gemm_batched(int nbatch, int n, const int* nsize, int transa, int transb, double alpha, const double* A,
+                      const double* B, const double* scale, double beta, const double* Cin, double* Out, int sym,
+                      int fill_pad, double pad, hipStream_t stream);
+
 /* Largest matrix order handled by dxtb_jacobi_eigh. */
 int dxtb_jacobi_max_n(void);
 
