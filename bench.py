@@ -223,27 +223,46 @@ def canonical_eigh_flops(n: int) -> float:
     return (4.0 / 3.0 + 2.0) * float(n) ** 3
 
 
-def jacobi_roofline(records) -> dict:
-    """Roofline of the eigensolver calls recorded in the timed region (HIP
-    events on each call's launch stream).  achieved = the canonical eigensolve
-    flops of every matrix solved (canonical_eigh_flops) / the summed event
-    time, against the FP64 matrix peak.  The HBM view of the same calls (the
-    bytes of the sweeps each matrix ran, and the PMC-measured traffic) is
-    reported beside it: at full occupancy the blocked kernels are bound by
-    HBM traffic (see the eigensolver microbenchmark).  With several pool
-    streams in flight an event's time includes time shared with other
-    streams' kernels, so both rates are lower bounds."""
+def _union_ms(intervals) -> float:
+    """Total length of the union of (start, end) intervals."""
+    tot, cs, ce = 0.0, None, None
+    for a, b in sorted(intervals):
+        if cs is None:
+            cs, ce = a, b
+        elif a <= ce:
+            ce = max(ce, b)
+        else:
+            tot += ce - cs
+            cs, ce = a, b
+    if cs is not None:
+        tot += ce - cs
+    return tot
+
+
+def jacobi_roofline(records, base) -> dict:
+    """Roofline of the eigensolver calls recorded in the timed region.  Each
+    call is bracketed by HIP events on its launch stream; ``base`` is an event
+    recorded at the start of the timed region, so every call becomes an
+    interval on one clock, and the eigensolver's busy time is the union of
+    those intervals (it cannot exceed the step, however the pools' streams
+    overlap).  achieved = the canonical eigensolve flops of every matrix
+    solved (canonical_eigh_flops) / that busy time, against the FP64 matrix
+    peak; per launch: the flops of one call / (busy time / calls).  The HBM
+    view of the same calls (the bytes of the sweeps each matrix ran, and the
+    PMC-measured traffic) is reported beside it.  Kernels of other pools that
+    run while an eigensolver call is in flight count as eigensolver time, so
+    the rates are lower bounds."""
     if not records:
         return None
-    t_ms = []
-    flops = 0.0
+    iv = []
     canon = 0.0
+    flops = 0.0
     nbytes = 0.0
     traffic = 0.0
     nsweep, nmat = 0, 0
     for e0, e1, n, nao, sw, variant, _split in records:
-        ms, sweeps = e0.elapsed_time(e1), sw.cpu().numpy()
-        t_ms.append(ms)
+        iv.append((base.elapsed_time(e0), base.elapsed_time(e1)))
+        sweeps = sw.cpu().numpy()
         nsweep += int(sweeps.sum())
         nmat += len(sweeps)
         for i in range(len(sweeps)):
@@ -253,7 +272,9 @@ def jacobi_roofline(records) -> dict:
                 flops += k * jacobi_flops_per_sweep(int(nao[i]), variant)
                 nbytes += k * jacobi_bytes(int(nao[i]), variant)
                 traffic += JACOBI_TRAFFIC_RATIO[variant] * k * jacobi_bytes(int(nao[i]), variant)
-    total_s = float(np.sum(t_ms)) * 1e-3
+    busy_ms = _union_ms(iv)
+    total_s = busy_ms * 1e-3
+    ncall = len(iv)
     achieved = canon / total_s / 1e12
     hbm = nbytes / total_s / 1e9
     return {
@@ -263,15 +284,16 @@ def jacobi_roofline(records) -> dict:
         "peak": FP64_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / FP64_PEAK_TFLOPS,
-        "traffic": traffic / len(t_ms),
-        "canonical_flops_per_launch": canon / len(t_ms),
-        "hbm_algorithmic_bytes_per_launch": nbytes / len(t_ms),
+        "traffic": traffic / ncall,
+        "canonical_flops_per_launch": canon / ncall,
+        "avg_launch_ms": busy_ms / ncall,
+        "launches": ncall,
+        "busy_ms": busy_ms,
+        "summed_event_ms": float(sum(b - a for a, b in iv)),
+        "hbm_algorithmic_bytes_per_launch": nbytes / ncall,
         "hbm_achieved_gbs": hbm,
         "hbm_frac": hbm / HBM_PEAK_GBS,
         "jacobi_sweep_fp64_tflops": flops / total_s / 1e12,
-        "launches": len(t_ms),
-        "avg_launch_ms": total_s * 1e3 / len(t_ms),
-        "total_kernel_ms": total_s * 1e3,
         "mean_sweeps": nsweep / max(nmat, 1),
     }
 
@@ -411,6 +433,9 @@ def bench_gfn2_batch(args, dev):
     torch.cuda.synchronize(dev)
     walls, out = [], []
     hscf.JACOBI_PROFILE = []
+    torch.cuda.synchronize(dev)
+    base = torch.cuda.Event(enable_timing=True)
+    base.record(torch.cuda.current_stream(dev))
     for _ in range(args.steps):
         eng = engine()
         reqs = _requests(nums, poss, chg, range(len(nums)))
@@ -420,7 +445,8 @@ def bench_gfn2_batch(args, dev):
         torch.cuda.synchronize(dev)
         walls.append(time.perf_counter() - t0)
         del eng
-    roof = jacobi_roofline(hscf.JACOBI_PROFILE)
+    roof = jacobi_roofline(hscf.JACOBI_PROFILE, base)
+    roof["busy_fraction_of_steps"] = roof["busy_ms"] * 1e-3 / float(np.sum(walls))
     hscf.JACOBI_PROFILE = None
     wall = float(np.mean(walls))
     lat = np.array([r.latency for r in out])
@@ -446,37 +472,38 @@ def bench_gfn2_batch(args, dev):
     return res, (nums, poss, chg, "config-2")
 
 
-def bench_serving(args, dev):
-    """Config 3: serving stream of seeded requests (log-normal sizes, median
-    40 atoms, 5-400; H/C/N/O/F/S/Cl; charges 0/+1/-1 at 80/10/10) with
-    Poisson arrivals at three offered loads (30/60/90 % of the capacity the
-    engine shows on an offline batch of the same distribution); compared
-    with the plain one-molecule-per-call GPU path."""
+def serving_stream(args, dev, nreq: int) -> tuple[dict, dict | None, tuple]:
+    """Config 3 on ``nreq`` seeded requests of the stream (log-normal sizes,
+    median 40 atoms, 5-400; H/C/N/O/F/S/Cl; charges 0/+1/-1 at 80/10/10; 20 %
+    with GBSA, eps 78.4): the offline capacity of the engine on the first
+    third, then Poisson arrivals at 30/60/90 % of that capacity on three
+    further slices, and the plain one-molecule-per-call GPU path on a sample.
+    Returns (results, eigensolver roofline of the streams, workload)."""
     from dxtb_amd import scf as hscf
     from dxtb_amd import serving as sv
     from dxtb_amd import workloads
 
-    nums, poss, chg = workloads.config_batch(3, count=args.nmol)
+    nums, poss, chg = workloads.config_batch(3, count=nreq)
     solv = workloads.config3_solvation(len(nums))
     n = len(nums)
     eng = sv.Engine("gfn2", dev, opts=hscf.SCFOptions(jacobi_variant=args.jacobi_variant),
                     graphs=args.engine_graphs)
-    for _ in range(args.warmup):
-        eng.run(_requests(nums, poss, chg, _stratified(nums, "gfn2", 4), solv), realtime=False,
-                progress=_progress("warm-up"))
-    # capacity: offline batch of the first third of the requests
+    # warm-up: a few molecules of every pool class the stream uses
+    eng.run(_requests(nums, poss, chg, _stratified(nums, "gfn2", 4), solv), realtime=False,
+            progress=_progress("config-3 warm-up"))
     cap_idx = range(0, n // 3)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     cap_out = eng.run(_requests(nums, poss, chg, cap_idx, solv), realtime=False,
-                      progress=_progress("capacity batch"))
+                      progress=_progress("config-3 capacity batch"))
     capacity = len(cap_out) / (time.perf_counter() - t0)
-    # streams at three offered loads, each on its own slice of the requests
     rng = np.random.default_rng(workloads.seed(3) + 1)
     per = (n - n // 3) // 3
     loads = []
     hscf.JACOBI_PROFILE = []
-    walls = []
+    torch.cuda.synchronize(dev)
+    base = torch.cuda.Event(enable_timing=True)
+    base.record(torch.cuda.current_stream(dev))
     for j, frac in enumerate((0.3, 0.6, 0.9)):
         idx = range(n // 3 + j * per, n // 3 + (j + 1) * per)
         reqs = _requests(nums, poss, chg, idx, solv)
@@ -486,17 +513,15 @@ def bench_serving(args, dev):
             r.arrival = float(a)
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
-        out = eng.run(reqs, realtime=True, progress=_progress(f"stream at {frac:.0%} load"))
+        out = eng.run(reqs, realtime=True, progress=_progress(f"config-3 stream at {frac:.0%} load"))
         wall = time.perf_counter() - t0
-        walls.append(wall)
         lat = np.array([r.latency for r in out])
         loads.append({"offered_mol_s": rate, "achieved_mol_s": len(out) / wall, "requests": len(out),
-                      "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
+                      "wall_s": wall, "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
                       "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
                       "converged": int(sum(r.converged for r in out))})
-    roof = jacobi_roofline(hscf.JACOBI_PROFILE)
+    roof = jacobi_roofline(hscf.JACOBI_PROFILE, base)
     hscf.JACOBI_PROFILE = None
-    # plain path: one molecule per call, on a sample of the same stream
     sample = _requests(nums, poss, chg, range(n // 3, n // 3 + args.plain_sample), solv)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
@@ -505,42 +530,88 @@ def bench_serving(args, dev):
         if k % 8 == 7:
             print(f"plain path: {k + 1}/{len(sample)}", file=sys.stderr, flush=True)
     plain = len(sample) / (time.perf_counter() - t0)
-    top = loads[-1]
+    res = {
+        "workload": f"config-3 serving stream: {n} seeded requests of the 100000-request stream (log-normal sizes, "
+                    "median 40 atoms, 5-400, H/C/N/O/F/S/Cl, charges 0/+-1, 20% GBSA eps 78.4); offline capacity "
+                    f"on the first {n // 3}, Poisson arrivals at 30/60/90% of it on three slices of {per}",
+        "requests": n,
+        "requests_note": "a prefix of the seeded stream: p50/p99 of a slice of ~400 requests are stable to a few "
+                         "percent between slices, while the full 100000 requests would take ~30 min per load",
+        "offline_capacity_mol_s": capacity,
+        "loads": loads,
+        "p99_latency_ms_at_90pct": loads[-1]["p99_latency_ms"],
+        "gbsa_requests": int(solv.sum()),
+        "plain_path_mol_s": plain,
+        "plain_sample": len(sample),
+        "serving_over_plain": capacity / plain,
+    }
+    return res, roof, (nums, poss, chg, "config-3", solv)
+
+
+def bench_serving(args, dev):
+    """Config 3 on its own (--workload serving): the serving stream's numbers
+    as the line's value (throughput at the 90 % load)."""
+    res3, roof, extra = serving_stream(args, dev, args.nmol)
+    top = res3["loads"][-1]
     res = {
         "metric": "GFN2 energy+forces molecules/s, p50/p99 latency at |dE|<1e-8 Eh",
         "value": top["achieved_mol_s"],
         "unit": "molecules/s",
-        "ms_per_step": walls[-1] * 1e3,
+        "ms_per_step": top["wall_s"] * 1e3,
         "dtype": "f64",
-        "config": {"workload": f"config-3 serving stream: {n} seeded requests (log-normal sizes, median 40 atoms, "
-                               "5-400, H/C/N/O/F/S/Cl, charges 0/+-1), Poisson arrivals at 30/60/90% of the "
-                               "offline capacity; 20% of requests with GBSA (eps 78.4); value = throughput at the 90% load",
-                   "offline_capacity_mol_s": capacity,
-                   "loads": loads,
-                   "p50_latency_ms": top["p50_latency_ms"],
-                   "p99_latency_ms": top["p99_latency_ms"],
-                   "gbsa_requests": int(solv.sum()),
-                   "plain_path_mol_s": plain,
-                   "plain_sample": len(sample),
-                   "serving_over_plain": capacity / plain},
+        "config": dict(res3, p50_latency_ms=top["p50_latency_ms"], p99_latency_ms=top["p99_latency_ms"]),
         "roofline": roof,
     }
-    return res, (nums, poss, chg, "config-3")
+    return res, extra
 
 
-def cpu_baseline_gfn2(nums, poss, chg, label, budget_s=20.0):
+def _spread_order(nums) -> list[int]:
+    """Molecule indices ordered so that every prefix spreads over the size
+    range: rank quantiles in van der Corput order (1/2, 1/4, 3/4, 1/8, ...)
+    of the molecules sorted by atom count."""
+    by_size = np.argsort([len(z) for z in nums], kind="stable")
+    m = len(by_size)
+    out, seen = [], set()
+    k = 1
+    while len(out) < m:
+        q, f, x = k, 0.5, 0.0
+        while q:
+            if q & 1:
+                x += f
+            q >>= 1
+            f *= 0.5
+        i = int(by_size[min(int(x * m), m - 1)])
+        if i not in seen:
+            seen.add(i)
+            out.append(i)
+        k += 1
+        if k > 64 * m:
+            out += [int(j) for j in by_size if int(j) not in seen]
+            break
+    return out
+
+
+def cpu_baseline_gfn2(nums, poss, chg, label, solv=None, budget_s=20.0):
+    """The FP64 CPU oracle (kind "port": torch on the host cores, LAPACK eigh,
+    xitorch-style broyden1 at the GPU path's tolerance 1e-8, variational
+    autograd forces) on a size-spread sample of the workload for about
+    ``budget_s`` seconds."""
     from oracle import scf as oscf
 
-    torch.set_num_threads(os.cpu_count() or 1)
+    nthreads = os.cpu_count() or 1
+    torch.set_num_threads(nthreads)
+    order = _spread_order(nums)
     t0 = time.perf_counter()
-    k = 0
-    per = []
-    while k < len(nums) and time.perf_counter() - t0 < budget_s:
+    per, atoms = [], []
+    for i in order:
+        if time.perf_counter() - t0 >= budget_s:
+            break
+        sv = {"eps": 78.4, "alpb": False, "kernel": "p16"} if solv is not None and solv[i] else None
         t1 = time.perf_counter()
-        oscf.singlepoint(nums[k].tolist(), torch.tensor(poss[k], dtype=torch.float64), "gfn2", charge=float(chg[k]),
-                         tol=1e-10, forces=True)
+        oscf.singlepoint(nums[i].tolist(), torch.tensor(poss[i], dtype=torch.float64), "gfn2", charge=float(chg[i]),
+                         tol=1e-8, maxiter=250, forces=True, solvation=sv)
         per.append(time.perf_counter() - t1)
-        k += 1
+        atoms.append(len(nums[i]))
     dt = time.perf_counter() - t0
     model = ""
     try:
@@ -548,16 +619,20 @@ def cpu_baseline_gfn2(nums, poss, chg, label, budget_s=20.0):
             model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
     except OSError:
         pass
+    k = len(per)
     return {
         "value": k / dt,
         "unit": "molecules/s",
         "cores": torch.get_num_threads(),
         "kind": "port",
-        "sample": f"first {k} {label} molecules (seeded), GFN2 energy+forces, FP64 torch CPU oracle "
-                  "(xitorch-style broyden1, LAPACK eigh, variational autograd forces)",
+        "sample": f"{k} {label} molecules spread over the size range (atom counts {min(atoms)}-{max(atoms)}, "
+                  f"mean {np.mean(atoms):.0f}; the workload's mean {np.mean([len(z) for z in nums]):.0f}), "
+                  "GFN2 energy+forces, FP64 torch CPU oracle (xitorch-style broyden1 at f_tol = x_tol = 1e-8 as "
+                  "on the GPU, LAPACK eigh, variational autograd forces)",
         "p50_ms_per_molecule": float(np.percentile(per, 50) * 1e3),
         "p99_ms_per_molecule": float(np.percentile(per, 99) * 1e3),
-        "nproc": os.cpu_count(),
+        "torch_threads": torch.get_num_threads(),
+        "nproc": nthreads,
         "cpu_model": model,
     }
 
@@ -571,6 +646,8 @@ def main():
     ap.add_argument("--eig-n", type=int, default=240)
     ap.add_argument("--eig-variant", type=int, default=2, choices=[2, 3])
     ap.add_argument("--nmol", type=int, default=None)
+    ap.add_argument("--serving-requests", type=int, default=1200,
+                    help="requests of the config-3 stream run once after the config-2 steps (0: skip)")
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--engine-graphs", action="store_true",
@@ -579,12 +656,13 @@ def main():
                     help="eigensolver form above order 64: 3 split (default), 2 one workgroup per matrix, 0 auto")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU only (one process, one device); --gpus must be 1"}))
         sys.exit(2)
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU available"}))
         sys.exit(1)
     dev = torch.device("cuda:0")
+    extra3 = None
     if args.workload == "integrals":
         args.nmol = args.nmol or 512
         res, extra = bench_integrals(args, dev)
@@ -598,6 +676,10 @@ def main():
     else:
         args.nmol = args.nmol or 1024
         res, extra = bench_gfn2_batch(args, dev)
+        if args.serving_requests > 0:
+            res3, roof3, extra3 = serving_stream(args, dev, args.serving_requests)
+            res3["roofline_eigensolver"] = roof3
+            res["config"]["config3_serving"] = res3
     res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "data": "synthetic (seeded)"})
     if not args.no_cpu_baseline:
@@ -605,6 +687,11 @@ def main():
             res["cpu_baseline"] = cpu_baseline_integrals(extra[0], extra[1])
         else:
             res["cpu_baseline"] = cpu_baseline_gfn2(*extra)
+        if extra3 is not None:
+            cpu3 = cpu_baseline_gfn2(*extra3, budget_s=15.0)
+            c3 = res["config"]["config3_serving"]
+            c3["cpu_baseline"] = cpu3
+            c3["serving_over_cpu"] = c3["offline_capacity_mol_s"] / cpu3["value"]
     print(json.dumps(res), flush=True)
 
 

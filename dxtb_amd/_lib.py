@@ -96,6 +96,8 @@ _SIGNATURES: dict[str, list] = {
     "dxtb_jacobi_eigh": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                          c_double, c_void_p, c_void_p, c_void_p],
     "dxtb_forces_classical": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 19,
+    "dxtb_broyden_work_size": [c_int, c_int],
+    "dxtb_broyden_step": [c_int, c_int, c_int] + [c_void_p] * 11 + [c_double] * 4 + [c_int, c_void_p],
     "dxtb_gemm_batched": [c_int, c_int, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_double,
                           c_void_p, c_void_p, c_int, c_int, c_double, c_void_p],
 }
@@ -125,7 +127,7 @@ def load() -> ctypes.CDLL:
     return _lib
 
 
-_RESTYPES = {"dxtb_jacobi_work_size": c_size_t}
+_RESTYPES = {"dxtb_jacobi_work_size": c_size_t, "dxtb_broyden_work_size": c_size_t}
 
 
 def check(rc: int, what: str) -> None:

@@ -628,45 +628,64 @@ def wnorm(v: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
     return torch.sqrt((w * v * v).sum(-1))
 
 
-class BatchedBroyden:
-    """Per-molecule xitorch-style broyden1 (good Broyden, rank-one updates of the
-    inverse Jacobian G = -alpha I + sum_k c_k <d_k, .>), with the inner product
-    <u, v> = sum_i w_i u_i v_i of the orbital-resolved potential vector (see
-    GeometryState.wvec): the iterates are those of broyden1 on the reference's
-    vector.  The update vectors live in preallocated (B, H, L) buffers so that
-    a product with G is two batched GEMVs whatever the history length."""
+class DeviceBroyden:
+    """Per-molecule broyden1 state of a batch on the device, advanced one
+    iteration at a time by ``dxtb_broyden_step`` (xitorch's broyden1 with the
+    weighted inner product of the orbital-resolved potential vector, see
+    GeometryState.wvec): the point to evaluate next (xe), the current point
+    and residual, the update history, and per molecule the control words
+    (occupied, started, history length, iterations), the stop flag (1
+    converged, 2 iteration limit) and the norm of the last step.  A molecule
+    starts at xe = 0 (the SAD guess: zero potential)."""
 
-    def __init__(self, alpha: float, damp: float, B: int, L: int, H: int, device, w: torch.Tensor):
-        self.g0 = -alpha
-        self.damp = damp
-        self.w = w
-        self.C = torch.zeros((B, H, L), dtype=torch.float64, device=device)
-        self.D = torch.zeros((B, H, L), dtype=torch.float64, device=device)
-        self.k = 0
+    def __init__(self, B: int, L: int, H: int, device):
+        f64 = dict(dtype=torch.float64, device=device)
+        self.L, self.H = L, H
+        self.xe = torch.zeros((B, L), **f64)
+        self.x = torch.zeros((B, L), **f64)
+        self.y = torch.zeros((B, L), **f64)
+        # entries beyond a molecule's history length are never read
+        self.hc = torch.empty((B, H, L), **f64)
+        self.hd = torch.empty((B, H, L), **f64)
+        self.ctl = torch.zeros((B, 4), dtype=torch.int32, device=device)
+        self.flag = torch.zeros(B, dtype=torch.int32, device=device)
+        self.dxn = torch.zeros(B, **f64)
+        self.work = torch.empty(B * 3 * L, **f64)
 
-    def _apply(self, A, Bm, v):
-        r = self.g0 * v
-        if self.k == 0:
-            return r
-        coef = torch.bmm(Bm[:, : self.k], (self.w * v).unsqueeze(-1))  # (B, k, 1)
-        return r + torch.bmm(A[:, : self.k].transpose(1, 2), coef).squeeze(-1)
+    _ROWS = ("xe", "x", "y", "hc", "hd", "ctl", "flag", "dxn")
 
-    def mv(self, v):
-        return self._apply(self.C, self.D, v)
+    def start(self, rows=None) -> None:
+        """Rows (None: all) restart as fresh molecules at the zero potential."""
+        if rows is None:
+            for t in (self.xe, self.x, self.y, self.dxn, self.flag):
+                t.zero_()
+            self.ctl.zero_()
+            self.ctl[:, 0] = 1
+            return
+        for t in (self.xe, self.x, self.y, self.dxn, self.flag, self.ctl):
+            t.index_fill_(0, rows, 0)
+        self.ctl[:, 0].index_fill_(0, rows, 1)
 
-    def rmv(self, v):
-        return self._apply(self.D, self.C, v)
+    def subset(self, keep: torch.Tensor) -> None:
+        for name in self._ROWS:
+            setattr(self, name, getattr(self, name).index_select(0, keep).contiguous())
+        self.work = self.work[: keep.numel() * 3 * self.L]
 
-    def update(self, ddx, ddy, active):
-        if self.k >= self.C.shape[1]:
-            # xitorch "restart" reduction: forget the history
-            self.k = 0
-        v = self.rmv(ddx)
-        den = (self.w * ddy * v).sum(-1, keepdim=True)
-        ok = active.unsqueeze(-1) & (den != 0)
-        self.C[:, self.k] = torch.where(ok, ddx - self.mv(ddy), 0.0)
-        self.D[:, self.k] = torch.where(ok, v / torch.where(den == 0, 1.0, den), 0.0)
-        self.k += 1
+    def move(self, src: torch.Tensor, dst: torch.Tensor) -> None:
+        """Copy rows src into rows dst (disjoint), in place."""
+        for name in self._ROWS:
+            t = getattr(self, name)
+            t.index_copy_(0, dst, t.index_select(0, src))
+
+    def step(self, gx: torch.Tensor, w: torch.Tensor, o: "SCFOptions", nb: int | None = None) -> None:
+        """One iteration of the first ``nb`` rows (default: all) with g(xe) = gx."""
+        nb = self.xe.shape[0] if nb is None else nb
+        lib = _lib.load()
+        rc = lib.dxtb_broyden_step(nb, self.L, self.H, _lib.ptr(w), _lib.ptr(gx.contiguous()), _lib.ptr(self.xe),
+                                   _lib.ptr(self.x), _lib.ptr(self.y), _lib.ptr(self.hc), _lib.ptr(self.hd),
+                                   _lib.ptr(self.ctl), _lib.ptr(self.flag), _lib.ptr(self.dxn), _lib.ptr(self.work),
+                                   o.alpha, o.damp, o.x_tol, o.f_tol, o.maxiter, _lib.stream_ptr(self.xe.device))
+        _lib.check(rc, "dxtb_broyden_step")
 
 
 @dataclass
@@ -689,7 +708,7 @@ class SCFResult:
 class BatchedFullMixer:
     """The reference's full-mode mixers, per molecule of a batch, on the
     potential vectors with the weighted inner product of the orbital-resolved
-    vector (as BatchedBroyden).
+    vector (as DeviceBroyden).
 
     "anderson" (``scf/mixer/anderson.py:163-317``): the history holds the
     previous inputs x_j and residuals F_j = g(x_j) - x_j (newest first, up to
@@ -768,9 +787,12 @@ class BatchedFullMixer:
 
 
 def run_scf(state: GeometryState, cull: float = 0.75) -> SCFResult:
-    """Iterate all molecules of ``state`` to self-consistency.  When the
-    active molecules drop below ``cull`` of the current batch, the batch is
-    compacted to the active ones (converged molecules are frozen).
+    """Iterate all molecules of ``state`` to self-consistency.  The mixer runs
+    on the device (DeviceBroyden); the host reads the stop flags one
+    iteration behind the device (pinned copies, double buffered), so it never
+    waits for the iteration it has just launched.  When the running
+    molecules drop below ``cull`` of the current batch, the batch is
+    compacted to them (stopped molecules are frozen on the device).
     ``state.opts.scf_mode`` picks the fixed-point form: implicit (broyden1)
     or full (Anderson / simple mixing)."""
     o = state.opts
@@ -780,52 +802,50 @@ def run_scf(state: GeometryState, cull: float = 0.75) -> SCFResult:
         raise ValueError(f"Unsupported scf_mode '{o.scf_mode}'.")
     B = state.bb.nbatch
     dev = state.dev
-    x_full = torch.zeros((B, state.nvec), dtype=torch.float64, device=dev)
+    f64 = dict(dtype=torch.float64, device=dev)
+    x_full = torch.zeros((B, state.nvec), **f64)
+    it_full = torch.zeros(B, dtype=torch.int32, device=dev)
+    flag_full = torch.zeros(B, dtype=torch.int32, device=dev)
     idx = torch.arange(B, device=dev)  # full-batch index of every current row
     cur = state
-    x = x_full.clone()
-    adaptive = o.adaptive_eigensolver
-    if adaptive:
-        cur._jtol = torch.full((B,), JACOBI_TOL_LOOSE, dtype=torch.float64, device=dev)
-    y = x - cur.g(x)
-    mix = BatchedBroyden(o.alpha, o.damp, B, state.nvec, o.maxiter + 1, dev, state.wvec)
-    active = torch.ones(B, dtype=torch.bool, device=dev)
-    iters = np.zeros(B, dtype=np.int64)
-    conv = np.zeros(B, dtype=bool)
-    xprev, yprev = x, y
-    for it in range(o.maxiter):
-        dx = -mix.mv(y)
-        dxn = wnorm(dx, cur.wvec)
-        xnew = torch.where(active.unsqueeze(-1), x + o.damp * dx, x)
-        if adaptive:
-            cur._jtol = jacobi_tolerance(wnorm(y, cur.wvec))
-        ynew = xnew - cur.g(xnew)
-        ynew = torch.where(active.unsqueeze(-1), ynew, y)
-        yn = wnorm(ynew, cur.wvec)
-        mix.update(xnew - xprev, ynew - yprev, active)
-        xprev, yprev = xnew, ynew
-        done = active & (dxn < o.x_tol) & (yn < o.f_tol)
-        x, y = xnew, ynew
-        a_host = active.cpu().numpy()
-        d_host = done.cpu().numpy()
-        rows = idx.cpu().numpy()
-        iters[rows[a_host]] = it + 1
-        conv[rows[d_host]] = True
-        active = active & ~done
-        if d_host.any():
-            x_full[idx[done]] = x[done]
-        n_act = int(active.sum())
-        if n_act == 0:
+    mix = DeviceBroyden(B, state.nvec, o.maxiter + 1, dev)
+    mix.start()
+    hflags = [torch.zeros(B, dtype=torch.int32, pin_memory=True) for _ in range(2)]
+    events: list = [None, None]
+    for it in range(o.maxiter + 1):
+        if o.adaptive_eigensolver:
+            started = mix.ctl[:, 1] != 0
+            cur._jtol = torch.where(started, jacobi_tolerance(wnorm(mix.y, cur.wvec)), JACOBI_TOL_LOOSE).contiguous()
+        mix.step(cur.g(mix.xe), cur.wvec, o)
+        nb = cur.bb.nbatch
+        hflags[it % 2][:nb].copy_(mix.flag, non_blocking=True)
+        events[it % 2] = torch.cuda.Event()
+        events[it % 2].record(torch.cuda.current_stream(dev))
+        prev = events[(it - 1) % 2]
+        if prev is None:
+            continue
+        prev.synchronize()  # flags after the previous iteration
+        fl = hflags[(it - 1) % 2][:nb].numpy()
+        running = fl == 0
+        n_run = int(running.sum())
+        if n_run == 0:
             break
-        if n_act < cull * cur.bb.nbatch:
-            keep = torch.nonzero(active).squeeze(-1)
+        if n_run < cull * nb:
+            keep = torch.as_tensor(np.nonzero(running)[0], device=dev)
+            drop = torch.as_tensor(np.nonzero(~running)[0], device=dev)
+            # stopped rows are frozen: their state is final
+            x_full.index_copy_(0, idx[drop], mix.x.index_select(0, drop))
+            it_full.index_copy_(0, idx[drop], mix.ctl[:, 3].index_select(0, drop))
+            flag_full.index_copy_(0, idx[drop], mix.flag.index_select(0, drop))
             cur = cur.subset(keep)
+            mix.subset(keep)
             idx = idx[keep]
-            x, y, xprev, yprev = x[keep], y[keep], xprev[keep], yprev[keep]
-            active = active[keep]
-            mix.C, mix.D, mix.w = mix.C[keep], mix.D[keep], cur.wvec
-    if bool(active.any()):
-        x_full[idx[active]] = x[active]
+            events = [None, None]  # the staged flags refer to the old rows
+    x_full.index_copy_(0, idx, mix.x)
+    it_full.index_copy_(0, idx, mix.ctl[:, 3].contiguous())
+    flag_full.index_copy_(0, idx, mix.flag)
+    iters = it_full.cpu().numpy().astype(np.int64)
+    conv = flag_full.cpu().numpy() == 1
     state._jtol = None  # the final solve of every molecule at the tight bar
     return finalize(state, x_full, iters, conv)
 

@@ -137,28 +137,16 @@ class SlotPool:
         nbytes = self.state.lib.dxtb_jacobi_work_size(capacity, nao_max)
         self.state._jwork = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=self.dev)  # shared by windows
         L = self.state.nvec
-        H = opts.maxiter + 1
-        f64 = dict(dtype=torch.float64, device=self.dev)
-        self.x = torch.zeros((capacity, L), **f64)
-        self.y = torch.zeros((capacity, L), **f64)
-        self.xprev = torch.zeros((capacity, L), **f64)
-        self.yprev = torch.zeros((capacity, L), **f64)
-        self.hc = torch.zeros((capacity, H, L), **f64)  # Broyden update vectors c_k
-        self.hd = torch.zeros((capacity, H, L), **f64)  # and d_k
-        self.k = np.zeros(capacity, dtype=np.int64)  # history length per slot
-        self.iters = np.zeros(capacity, dtype=np.int64)
+        # per-slot Broyden state on the device (potential vectors, history,
+        # control words, stop flags): an iteration needs no host data
+        self.mix = hscf.DeviceBroyden(capacity, L, opts.maxiter + 1, self.dev)
         self.occupied = np.zeros(capacity, dtype=bool)
-        self.fresh = np.zeros(capacity, dtype=bool)  # admitted, first map evaluation pending
         self.req: list[Request | None] = [None] * capacity
-        # per-slot control of the iteration (occupied, started, history index)
-        # and its convergence flags, at fixed addresses for graph replay
-        self.d_ctl = torch.zeros((3, capacity), dtype=torch.int64, device=self.dev)
-        self.d_done = torch.zeros(capacity, dtype=torch.bool, device=self.dev)
-        # pinned staging of both, reused every iteration: an iteration is
-        # launched only after the previous one (and so its copies) completed
-        self.h_ctl = torch.zeros(3 * capacity, dtype=torch.int64, pin_memory=True)
-        self.h_done = torch.zeros(capacity, dtype=torch.bool, pin_memory=True)
-        self.d_ar = torch.arange(capacity, device=self.dev)
+        # pinned staging of the stop flags and iteration counts, reused every
+        # iteration: an iteration is launched only after the previous one (and
+        # so its copies) completed
+        self.h_flag = torch.zeros(capacity, dtype=torch.int32, pin_memory=True)
+        self.h_iter = torch.zeros(capacity, dtype=torch.int32, pin_memory=True)
         self._wins: dict = {}
         # graph mode needs a capturable eigensolver (the Jacobi kernels)
         self.graphs = graphs and nao_max <= self.state.lib.dxtb_jacobi_max_n()
@@ -206,12 +194,8 @@ class SlotPool:
         rows64 = np.asarray(rows, dtype=np.int64)
         rt = _to_device(rows64, self.dev)
         self.state.scatter(rows64, st)
-        for t in (self.x, self.y, self.xprev, self.yprev, self.hc, self.hd):
-            t.index_fill_(0, rt, 0.0)
-        self.k[rows] = 0
-        self.iters[rows] = 0
+        self.mix.start(rt)
         self.occupied[rows] = True
-        self.fresh[rows] = True
         for r, row in zip(reqs, rows):
             r.admitted = now
             self.req[row] = r
@@ -230,26 +214,14 @@ class SlotPool:
         dst = np.nonzero(~self.occupied[:n])[0]
         self.state.move(src, dst)
         st, dt = _to_device(src, self.dev), _to_device(dst, self.dev)
-        for t in (self.x, self.y, self.xprev, self.yprev, self.hc, self.hd):
-            t.index_copy_(0, dt, t.index_select(0, st))
-        for a in (self.k, self.iters, self.occupied, self.fresh):
-            a[dst] = a[src]
+        self.mix.move(st, dt)
+        self.mix.ctl[:, 0].index_fill_(0, st, 0)  # the source slots are free
+        self.occupied[dst] = True
         self.occupied[src] = False
-        self.fresh[src] = False
         for s_, d_ in zip(src, dst):
             self.req[d_], self.req[s_] = self.req[s_], None
         self.moves += len(src)
         return True
-
-    # ---- one SCF iteration of every occupied slot --------------------------
-    def _mv(self, A, B, v, kmax, w):
-        """(g0 + sum_k a_k <b_k, .>_w) v with the weighted inner product of
-        the orbital-resolved potential vector (scf.BatchedBroyden)."""
-        r = -self.opts.alpha * v
-        if kmax == 0:
-            return r
-        coef = torch.bmm(B[:, :kmax], (w * v).unsqueeze(-1))
-        return r + torch.bmm(A[:, :kmax].transpose(1, 2), coef).squeeze(-1)
 
     # ---- one SCF iteration of the occupied window --------------------------
     def _graphed(self, hi: int) -> bool:
@@ -265,12 +237,10 @@ class SlotPool:
                 return w
         return self.cap
 
-    def _iterate(self, W: int, kmax: int) -> None:
-        """One iteration over rows [0, W): Broyden step of the started slots,
-        the SCF map of every row, Broyden update at each slot's own history
-        index, convergence flags into ``d_done``.  Reads the per-slot control
-        (occupied, started, history index) from ``d_ctl``; no host
-        synchronisation, fixed shapes (capturable)."""
+    def _iterate(self, W: int) -> None:
+        """One iteration over rows [0, W): the SCF map at every row's next
+        point, then the device Broyden step (dxtb_broyden_step), which skips
+        free and stopped slots.  No host data, fixed shapes (capturable)."""
         o = self.opts
         win = self._wins.get(W)
         if win is None:
@@ -278,38 +248,13 @@ class SlotPool:
         # the solvation flags of the pool change when a GBSA request is admitted
         # after the window was made (its tensors are views of the pool's rows)
         win.any_gb, win._alpb, win.gb_par = self.state.any_gb, self.state._alpb, self.state.gb_par
-        occ, started = self.d_ctl[0, :W].bool(), self.d_ctl[1, :W].bool()
-        kslot = self.d_ctl[2, :W].clamp(max=self.hc.shape[1] - 1)
-        x, y = self.x[:W], self.y[:W]
-        xprev, yprev = self.xprev[:W], self.yprev[:W]
-        hc, hd = self.hc[:W], self.hd[:W]
-        w = win.wvec
-        dx = -self._mv(hc, hd, y, kmax, w)
-        dxn = hscf.wnorm(dx, w)
-        xnew = torch.where(started.unsqueeze(-1), x + o.damp * dx, x)
+        m = self.mix
         if o.adaptive_eigensolver:  # fresh slots: a loose bar for the cold first solve
-            win._jtol = torch.where(started, hscf.jacobi_tolerance(hscf.wnorm(y, w)),
+            started = m.ctl[:W, 1] != 0
+            win._jtol = torch.where(started, hscf.jacobi_tolerance(hscf.wnorm(m.y[:W], win.wvec)),
                                     hscf.JACOBI_TOL_LOOSE).contiguous()
-        ynew = torch.where(occ.unsqueeze(-1), xnew - win.g(xnew), y)
-        yn = hscf.wnorm(ynew, w)
-        # Broyden update of the started slots at their own history index
-        ddx, ddy = xnew - xprev, ynew - yprev
-        v = self._mv(hd, hc, ddx, kmax, w)
-        den = (w * ddy * v).sum(-1, keepdim=True)
-        ok = started.unsqueeze(-1) & (den != 0)
-        cnew = torch.where(ok, ddx - self._mv(hc, hd, ddy, kmax, w), 0.0)
-        dnew = torch.where(ok, v / torch.where(den == 0, 1.0, den), 0.0)
-        ar = self.d_ar[:W]
-        keep = ~started.unsqueeze(-1)
-        hc[ar, kslot] = torch.where(keep, hc[ar, kslot], cnew)
-        hd[ar, kslot] = torch.where(keep, hd[ar, kslot], dnew)
-        # the new point becomes the current one for every occupied slot
-        upd = occ.unsqueeze(-1)
-        x.copy_(torch.where(upd, xnew, x))
-        y.copy_(torch.where(upd, ynew, y))
-        xprev.copy_(torch.where(upd, xnew, xprev))
-        yprev.copy_(torch.where(upd, ynew, yprev))
-        self.d_done[:W].copy_(started & (dxn < o.x_tol) & (yn < o.f_tol))
+        gx = win.g(m.xe[:W])
+        m.step(gx, win.wvec, o, nb=W)
 
     def launch(self):
         """Enqueue one iteration over the occupied window on the current
@@ -320,58 +265,43 @@ class SlotPool:
         if hi == 0:
             return None
         W = self._window(hi)
-        srows = np.nonzero(self.occupied[:hi] & ~self.fresh[:hi])[0]
-        ctl = np.zeros((3, W), dtype=np.int64)
-        ctl[0, :hi] = self.occupied[:hi]
-        ctl[1, :hi] = self.occupied[:hi] & ~self.fresh[:hi]
-        ctl[2, :hi] = self.k[:hi]
-        # every host-to-device copy comes before the iteration is enqueued
-        hv = self.h_ctl[: 3 * W].view(3, W)
-        hv.numpy()[:] = ctl
-        self.d_ctl[:, :W].copy_(hv, non_blocking=True)
         if self._graphed(hi):
             key = (W, self.state.any_gb, self.state._alpb)
             g = self._graphs.get(key)
             if g is None:
-                self._iterate(W, self.hc.shape[1])  # eager: warms up, and is this iteration
+                self._iterate(W)  # eager: warms up, and is this iteration
                 g = torch.cuda.CUDAGraph()
                 with torch.cuda.graph(g, pool=self._gpool, stream=torch.cuda.current_stream(self.dev)):
-                    self._iterate(W, self.hc.shape[1])
+                    self._iterate(W)
                 self._graphs[key] = g
             else:
                 g.replay()
         else:
-            self._iterate(W, int(self.k[:hi][self.occupied[:hi]].max(initial=0)))
-        done_h = self.h_done[:hi]
-        done_h.copy_(self.d_done[:hi], non_blocking=True)  # read by finish() once the pool's event completed
-        return hi, done_h, srows
+            self._iterate(W)
+        # read by finish() once the pool's event completed
+        self.h_flag[:hi].copy_(self.mix.flag[:hi], non_blocking=True)
+        self.h_iter[:hi].copy_(self.mix.ctl[:hi, 3], non_blocking=True)
+        return hi
 
     def finish(self, pending, now_fn) -> list[Request]:
-        """Host bookkeeping of a launched iteration; retires finished slots."""
+        """Host bookkeeping of a launched iteration; retires stopped slots
+        (converged, or at the iteration limit)."""
         if pending is None:
             return []
-        hi, done_h, srows = pending
-        done = done_h.numpy().copy()
-        self.iters[srows] += 1
-        self.k[srows] += 1
-        full = srows[self.k[srows] >= self.hc.shape[1]]
-        if len(full):  # history full: restart it (never reached with H = maxiter + 1)
-            ft = torch.as_tensor(full, device=self.dev)
-            self.hc.index_fill_(0, ft, 0.0)
-            self.hd.index_fill_(0, ft, 0.0)
-            self.k[full] = 0
-        self.fresh[:hi] = False
-        out = np.nonzero(done | (self.occupied[:hi] & (self.iters[:hi] >= self.opts.maxiter)))[0]
+        hi = pending
+        flag = self.h_flag[:hi].numpy().copy()
+        its = self.h_iter[:hi].numpy().astype(np.int64)
+        out = np.nonzero(self.occupied[:hi] & (flag != 0))[0]
         if len(out) == 0:
             return []
-        res = self.retire(out, done[out], now_fn)
+        res = self.retire(out, flag[out] == 1, its[out], now_fn)
         # a long-running molecule in a high slot would keep the window wide
         n = int(self.occupied.sum())
         if self.hi - n >= max(4, n // 4):
             self.compact()
         return res
 
-    def retire(self, rows: np.ndarray, conv: np.ndarray, now_fn) -> list[Request]:
+    def retire(self, rows: np.ndarray, conv: np.ndarray, iters: np.ndarray, now_fn) -> list[Request]:
         """Enqueue energy and forces of the finished slots ``rows`` on the
         pool's stream (on copies of their rows, so the slots are free at once)
         and copy the results to pinned host memory without waiting; the
@@ -384,7 +314,7 @@ class SlotPool:
         bb = build_batch(N, self.gfn, self.dev, self.dims)
         rows64 = np.asarray(rows, dtype=np.int64)
         sub = self.state.subset(rows64, bb=bb)
-        res = hscf.finalize(sub, self.x.index_select(0, _to_device(rows64, self.dev)), self.iters[rows].copy(),
+        res = hscf.finalize(sub, self.mix.x.index_select(0, _to_device(rows64, self.dev)), iters.copy(),
                             conv.copy())
         f = hscf.forces(res)
         e_h = torch.empty(res.energy.shape, dtype=torch.float64, pin_memory=True)
@@ -393,9 +323,8 @@ class SlotPool:
         f_h.copy_(f, non_blocking=True)
         ev = torch.cuda.Event()
         ev.record(torch.cuda.current_stream(self.dev))
-        self.retiring.append((ev, reqs, e_h, f_h, self.iters[rows].copy(), conv.copy(), (res, f)))
+        self.retiring.append((ev, reqs, e_h, f_h, iters.copy(), conv.copy(), (res, f)))
         self.occupied[rows] = False
-        self.fresh[rows] = False
         for r in rows:
             self.req[r] = None
         return self.collect(now_fn)

@@ -227,6 +227,31 @@ int dxtb_gemm_batched(int nbatch, int n, const int* nsize, int transa, int trans
                       const double* B, const double* scale, double beta, const double* Cin, double* Out, int sym,
                       int fill_pad, double pad, hipStream_t stream);
 
+/* Bytes of the workspace of dxtb_broyden_step (nbatch x 3 x L doubles). */
+size_t dxtb_broyden_work_size(int nbatch, int L);
+
+/* One iteration of the per-molecule Broyden mixer of the SCF, after the
+ * caller has evaluated the SCF map at xe (gx = g(xe)), for nbatch molecules
+ * with potential vectors of length L (row-major (nbatch, L) arrays):
+ * residual ynew = xe - gx; a fresh molecule (ctl started == 0) keeps x = xe,
+ * y = ynew; otherwise the first-Broyden rank-one update of its inverse
+ * Jacobian from ddx = xe - x, ddy = ynew - y (history (nbatch, H, L) in hc
+ * and hd, its length in ctl), x = xe, y = ynew, and the convergence test
+ * |dx| < x_tol and |ynew| < f_tol (dx the step that led to xe).  The next
+ * point xe = x - damp G ynew is written for every molecule that goes on.
+ * Inner products and norms are weighted by w (nbatch, L).
+ * ctl (nbatch, 4) int32: occupied, started, history length, iterations;
+ * a molecule with occupied == 0 is skipped; one that converges (flag 1) or
+ * reaches maxiter iterations (flag 2) gets occupied = 0.  dxn (nbatch): norm
+ * of the last step.  work: dxtb_broyden_work_size bytes.  H <= 4096.
+ * Replaces the broyden1 root finder of the reference's implicit SCF
+ * (exlibs/xitorch/_impls/optimize/root/rootsolver.py:42-261 with
+ * _jacobian.py:51-120, 250-316; alpha = -0.5, damp = 0.5 as set in
+ * scf/base.py:268-287), applied per molecule. */
+int dxtb_broyden_step(int nbatch, int L, int H, const double* w, const double* gx, double* xe, double* x, double* y,
+                      double* hc, double* hd, int* ctl, int* flag, double* dxn, double* work, double alpha,
+                      double damp, double x_tol, double f_tol, int maxiter, hipStream_t stream);
+
 /* Largest matrix order handled by dxtb_jacobi_eigh. */
 int dxtb_jacobi_max_n(void);
 

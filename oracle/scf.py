@@ -157,21 +157,24 @@ class System:
         return e, linv.T @ c
 
     def occupation(self, e):
-        """Fermi-Dirac occupations (2, nao) for the alpha/beta counts."""
+        """Fermi-Dirac occupations (2, nao) for the alpha/beta counts: the
+        chemical potential of each spin channel by bisection, both channels
+        at once, until the bracket cannot shrink any further (adjacent
+        doubles: the midpoint equals an end)."""
         kt = self.kt
-        occ = []
-        for n in self.nab:
-            lo, hi = e.min() - 1.0, e.max() + 1.0
-            for _ in range(200):
-                mu = 0.5 * (lo + hi)
-                nn = torch.special.expit(-(e - mu) / kt).sum()
-                if nn > n:
-                    hi = mu
-                else:
-                    lo = mu
+        nab = self.nab.to(e.dtype)
+        lo = torch.full_like(nab, float(e.min()) - 1.0)
+        hi = torch.full_like(nab, float(e.max()) + 1.0)
+        for _ in range(200):
             mu = 0.5 * (lo + hi)
-            occ.append(torch.special.expit(-(e - mu) / kt))
-        return torch.stack(occ)
+            if bool(((mu == lo) | (mu == hi)).all()):
+                break
+            nn = torch.special.expit(-(e.unsqueeze(0) - mu.unsqueeze(1)) / kt).sum(-1)
+            gt = nn > nab
+            hi = torch.where(gt, mu, hi)
+            lo = torch.where(gt, lo, mu)
+        mu = 0.5 * (lo + hi)
+        return torch.special.expit(-(e.unsqueeze(0) - mu.unsqueeze(1)) / kt)
 
     def solve_density(self, m, f):
         e, c = self.diagonalize(m, f)

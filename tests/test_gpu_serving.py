@@ -134,9 +134,9 @@ def test_compaction_and_admission_into_busy_pool(cuda):
 def test_compact_moves_slots_exactly(cuda):
     """Pool compaction: after three iterations of 8 molecules the five lowest
     slots are dropped; compact() moves the three survivors into slots 0-2
-    bit for bit (potential vectors, Broyden history, geometry state, basis
-    maps), and the moved molecules then converge to the one-molecule path's
-    results."""
+    bit for bit (potential vectors, Broyden history in use, control words,
+    geometry state, orbitals, basis maps), and the moved molecules then
+    converge to the one-molecule path's results."""
     nums, poss, chg = workloads.config_batch(3, count=400)
     cls = [sv.pool_class(*sv.sizes(z, "gfn2")) for z in nums]
     sel = [i for i in range(len(nums)) if cls[i] == 96][:8]
@@ -156,24 +156,31 @@ def test_compact_moves_slots_exactly(cuda):
     torch.cuda.synchronize()
     assert pool.occupied[:8].all(), "no molecule may converge in three iterations"
     keep = [5, 6, 7]
-    snap = {name: getattr(pool, name)[keep].clone() for name in ("x", "y", "xprev", "yprev", "hc", "hd")}
-    snap.update({name: getattr(pool.state, name)[keep].clone() for name in ("S", "H0", "U", "X", "pos")})
+    m = pool.mix
+    vec = ("xe", "x", "y", "ctl", "flag", "dxn")
+    snap = {name: getattr(m, name)[keep].clone() for name in vec}
+    hist = [int(k) for k in m.ctl[keep, 2].tolist()]  # history entries in use (the rest is never read)
+    assert min(hist) >= 1
+    snap_h = [(m.hc[s, :k].clone(), m.hd[s, :k].clone()) for s, k in zip(keep, hist)]
+    snap.update({name: getattr(pool.state, name)[keep].clone() for name in ("S", "H0", "U", "Corb", "X", "pos")})
     host = {k: getattr(pool.state.bb, k)[keep].copy() for k in ("numbers", "nao", "sh_l", "ao_shell")}
-    k_before = pool.k[keep].copy()
     pool.occupied[:5] = False
+    m.ctl[:5, 0] = 0
     for r in range(5):
         pool.req[r] = None
     with torch.cuda.stream(pool.stream):
         assert pool.compact()
     torch.cuda.synchronize()
     assert pool.occupied.tolist() == [True] * 3 + [False] * 5 and pool.hi == 3 and pool.moves == 3
-    for name in ("x", "y", "xprev", "yprev", "hc", "hd"):
-        assert torch.equal(getattr(pool, name)[:3], snap[name]), name
-    for name in ("S", "H0", "U", "X", "pos"):
+    for name in vec:
+        assert torch.equal(getattr(m, name)[:3], snap[name]), name
+    for d, (k, (hc, hd)) in enumerate(zip(hist, snap_h)):
+        assert torch.equal(m.hc[d, :k], hc) and torch.equal(m.hd[d, :k], hd)
+    assert (m.ctl[3:, 0] == 0).all()  # the source slots are free on the device too
+    for name in ("S", "H0", "U", "Corb", "X", "pos"):
         assert torch.equal(getattr(pool.state, name)[:3], snap[name]), name
     for k, v in host.items():
         assert np.array_equal(getattr(pool.state.bb, k)[:3], v), k
-    assert pool.k[:3].tolist() == k_before.tolist()
     assert [r.rid for r in pool.req[:3]] == [sel[i] for i in keep]
     out = []
     with torch.cuda.stream(pool.stream):
