@@ -18,11 +18,13 @@ LIBPATH = Path(__file__).resolve().parent / "libdxtb_hip.so"
 DXTB_EINVAL = 1001
 DXTB_EUNSUP = 1002
 DXTB_ENOCONV = 1003
+DXTB_ESCRATCH = 1004
 
 _ERRORS = {
     DXTB_EINVAL: "invalid argument",
     DXTB_EUNSUP: "unsupported option",
     DXTB_ENOCONV: "iterative solve did not converge",
+    DXTB_ESCRATCH: "request exceeds the fixed scratch buffer of the no-allocation entry point",
 }
 
 
@@ -95,6 +97,8 @@ _SIGNATURES: dict[str, list] = {
     "dxtb_jacobi_work_size": [c_int, c_int],
     "dxtb_jacobi_eigh": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                          c_double, c_void_p, c_void_p, c_void_p],
+    "dxtb_jacobi_eigh_noalloc": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                                 c_double, c_void_p, c_void_p, c_void_p],
     "dxtb_forces_classical": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 19,
     "dxtb_broyden_work_size": [c_int, c_int],
     "dxtb_broyden_step": [c_int, c_int, c_int] + [c_void_p] * 11 + [c_double] * 4 + [c_int, c_void_p],

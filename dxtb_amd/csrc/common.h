@@ -18,6 +18,7 @@ enum {
   DXTB_EINVAL = 1001,   // bad argument (null pointer, size out of range)
   DXTB_EUNSUP = 1002,   // unsupported angular momentum / option
   DXTB_ENOCONV = 1003,  // iterative solve did not converge
+  DXTB_ESCRATCH = 1004, // request exceeds the fixed scratch buffer of a no-allocation entry point
 };
 
 __device__ __forceinline__ double atomic_add_f64(double* p, double v) {

@@ -919,6 +919,39 @@ hipError_t launch_graph(hipStream_t st, const GraphKey& k, const Layout& w) {
 
 }  // namespace
 
+// Workspace of dxtb_jacobi_eigh_noalloc: one copy of this array per GPU, held
+// until the process ends.  Host code never passes the array's name to a
+// kernel (in host code it is a host-side stand-in): it uses the device address
+// that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[DXTB_JACOBI_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double* scratch_addr[MAX_DEVICES];
+
+// Device address of this GPU's scratch array, looked up once per device (the
+// device must be current).
+hipError_t scratch_for_device(int dev, double** out) {
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_addr[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (p == nullptr) return hipErrorInvalidSymbol;
+    scratch_addr[dev] = static_cast<double*>(p);
+  }
+  *out = scratch_addr[dev];
+  return hipSuccess;
+}
+
+int jacobi_impl(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals, double* work,
+                int variant, int max_sweeps, int split_sweeps, double tol, const double* tol_b, int* sweeps,
+                hipStream_t st, bool allow_graphs);
+
+}  // namespace
+
 extern "C" int dxtb_jacobi_max_n(void) { return JMAX; }
 
 extern "C" size_t dxtb_jacobi_work_size(int nbatch, int nmax) {
@@ -931,6 +964,31 @@ extern "C" size_t dxtb_jacobi_work_size(int nbatch, int nmax) {
 extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals,
                                 double* work, int variant, int max_sweeps, int split_sweeps, double tol,
                                 const double* tol_b, int* sweeps, hipStream_t st) {
+  return jacobi_impl(nbatch, nmax, nsize, A, VT, evals, work, variant, max_sweeps, split_sweeps, tol, tol_b, sweeps,
+                     st, true);
+}
+
+extern "C" int dxtb_jacobi_eigh_noalloc(int nbatch, int nmax, const int* nsize, double* A, double* VT,
+                                        double* evals, int variant, int max_sweeps, int split_sweeps, double tol,
+                                        const double* tol_b, int* sweeps, hipStream_t st) {
+  if (nbatch <= 0 || nmax <= 0 || nmax > JMAX) return DXTB_EINVAL;
+  if (dxtb_jacobi_work_size(nbatch, nmax) > DXTB_JACOBI_SCRATCH_BYTES) return DXTB_ESCRATCH;
+  int dev = 0;
+  DXTB_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return DXTB_EINVAL;
+  double* work = nullptr;
+  DXTB_CHECK(scratch_for_device(dev, &work));
+  // direct launches only: recording a launch sequence as a graph is not for
+  // callers that must not allocate
+  return jacobi_impl(nbatch, nmax, nsize, A, VT, evals, work, variant, max_sweeps, split_sweeps, tol, tol_b, sweeps,
+                     st, false);
+}
+
+namespace {
+
+int jacobi_impl(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals, double* work,
+                int variant, int max_sweeps, int split_sweeps, double tol, const double* tol_b, int* sweeps,
+                hipStream_t st, bool allow_graphs) {
   if (nbatch <= 0 || nmax <= 0 || nmax > JMAX || !nsize || !A || !VT || !evals || max_sweeps < 0 || variant < 0 ||
       variant > 3 || split_sweeps < 0)
     return DXTB_EINVAL;
@@ -968,10 +1026,13 @@ extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* 
   // the recorded sequence needs a stream that can be captured: not the null
   // stream, and not one that the caller is capturing already
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (split && st != nullptr && graphs_enabled()) DXTB_CHECK(hipStreamIsCapturing(st, &cs));
-  if (split && st != nullptr && graphs_enabled() && cs == hipStreamCaptureStatusNone) {
+  const bool graphs = allow_graphs && split && st != nullptr && graphs_enabled();
+  if (graphs) DXTB_CHECK(hipStreamIsCapturing(st, &cs));
+  if (graphs && cs == hipStreamCaptureStatusNone) {
     const GraphKey k{dev, nbcap, nmax, nsw, max_sweeps, tol, work};
     return (int)launch_graph(st, k, w);
   }
   return (int)enqueue_blocked(st, w, nbcap, nmax, nsw, max_sweeps, tol);
 }
+
+}  // namespace

@@ -28,6 +28,7 @@ extern "C" {
 #define DXTB_EINVAL 1001
 #define DXTB_EUNSUP 1002
 #define DXTB_ENOCONV 1003
+#define DXTB_ESCRATCH 1004 /* request exceeds the fixed scratch buffer of a no-allocation entry point */
 
 /* Device-resident description of a padded batch of molecules and its
  * shell-pair work lists (built once per set of atomic numbers). */
@@ -206,6 +207,25 @@ int dxtb_gb_potential(int nbatch, int nat_max, const int* nat, const double* pos
 int dxtb_gb_forces(int nbatch, int nat_max, const int* nat, const int* numbers, const double* pos, const double* keps,
                    const int* kind, const double* rvdw_z, const dxtb_gb_params_t* par, const double* born,
                    const double* dborn, const double* qat, double* grad, hipStream_t stream);
+
+/* Fixed device workspace of dxtb_jacobi_eigh_noalloc, per GPU (64 MiB). */
+#define DXTB_JACOBI_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* dxtb_jacobi_eigh without a workspace argument and without any device
+ * allocation: the workspace is a fixed array of DXTB_JACOBI_SCRATCH_BYTES in
+ * device memory, one per GPU, kept until the process ends; the split form's
+ * launches are always issued directly (no recorded graphs).  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Arguments and results as dxtb_jacobi_eigh.  A call whose
+ * workspace (dxtb_jacobi_work_size(nbatch, nmax)) exceeds the array returns
+ * DXTB_ESCRATCH and launches nothing, e.g. beyond about 1000 matrices of
+ * order 256 or 160 of order 1536.  All calls on one GPU share its array, so
+ * calls that overlap on one GPU (different streams, or threads) race: order
+ * them on one stream. */
+int dxtb_jacobi_eigh_noalloc(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals,
+                             int variant, int max_sweeps, int split_sweeps, double tol, const double* tol_b,
+                             int* sweeps, hipStream_t stream);
 
 /* Batched FP64 product on the matrix cores, for every matrix b of a batch of
  * padded (n, n) row-major matrices:
