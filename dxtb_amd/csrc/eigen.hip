@@ -495,11 +495,134 @@ __global__ void k_bj_args(JArgs* __restrict__ ja, JArgs v) {
   if (threadIdx.x == 0) *ja = v;
 }
 
+// Cross sweep of block pair k at outer step s > 0 with the subproblem and its
+// rotation in registers.  Round r (< 16) rotates the pairs (I_x, J_sigma(x)),
+// sigma(x) = (x + r) mod 16, for every slot x < 16.  In slot space the four
+// 16x16 blocks are
+//   aii[j][x] = A(I_j, I_x), bij[j][x] = A(I_j, J_sigma(x)),
+//   bji[j][x] = A(J_sigma(j), I_x), bjj[j][x] = A(J_sigma(j), J_sigma(x)),
+// so the 2x2 tile touched by the rotations of pairs j (rows) and x (columns)
+// sits at (j, x) of all four; likewise Q's rows i hold Q(i, I_x) and
+// Q(i, J_sigma(x)) at slot x.  Lane l owns slot column x = l % 16 and the row
+// slots j = 4 g + u (g = l / 16, u < 4) of the tiles, and the Q rows
+// i = 8 g + v (v < 8).  Going from round r to r + 1 advances sigma by one:
+// the J side of every slot moves one slot down (column x takes the value of
+// x + 1, row j that of j + 1), by register moves and cross-lane shuffles.
+// After 16 rounds sigma is the identity again.  The lane holding the diagonal
+// tile (j = x) computes the rotation of pair x; the coefficients are exchanged
+// through LDS (W.S as scratch).  Same rotations, in the same order, as the
+// cyclic cross sweep of inner_sweep.
+__device__ bool bj_inner_cross(WaveLds& W, const BjMat& M, int k, int s, double thr, int lane) {
+  int bi, bj;
+  block_pair(k, s, M.nb, bi, bj);
+  const int x = lane & 15, g = lane >> 4;
+  const int n = M.n, nmax = M.nmax, last = M.nlast;
+  const int gix = bi * BS + x, gjx = bj * BS + x;
+  double aii[4], bij[4], bji[4], bjj[4], qi[8], qj[8];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int j = 4 * g + u;
+    const int gij = bi * BS + j, gjj = bj * BS + j;
+    const double* Ai = M.A + (size_t)min(gij, last) * nmax;
+    const double* Aj = M.A + (size_t)min(gjj, last) * nmax;
+    aii[u] = Ai[min(gix, last)] * mask01(gij < n && gix < n);
+    bij[u] = Ai[min(gjx, last)] * mask01(gij < n && gjx < n);
+    bji[u] = Aj[min(gix, last)] * mask01(gjj < n && gix < n);
+    bjj[u] = Aj[min(gjx, last)] * mask01(gjj < n && gjx < n);
+  }
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    const int i = 8 * g + v;
+    qi[v] = (i == x) ? 1.0 : 0.0;
+    qj[v] = (i == BS + x) ? 1.0 : 0.0;
+  }
+  // this lane holds the diagonal tile (x, x) when x is one of its row slots
+  const int ud = x - 4 * g;
+  const bool diag = ud >= 0 && ud < 4;
+  const int nxt = (lane & 48) | ((lane + 1) & 15);  // slot x + 1, same row group
+  const int down = (((g + 1) & 3) << 4) | x;        // row slot 4 g + 4, same column
+  const int down_nxt = (((g + 1) & 3) << 4) | ((x + 1) & 15);
+  double* cs = W.S;  // (c, s) of the 16 pairs
+  bool any = false;
+  for (int r = 0; r < BS; ++r) {
+    double app = aii[0], aqq = bjj[0], apq = bij[0];
+#pragma unroll
+    for (int u = 1; u < 4; ++u) {
+      app = ud == u ? aii[u] : app;
+      aqq = ud == u ? bjj[u] : aqq;
+      apq = ud == u ? bij[u] : apq;
+    }
+    double c, sn;
+    const bool rot = rotation(app, aqq, apq, thr, c, sn) && diag;
+    if (__any(rot ? 1 : 0)) {
+      any = true;
+      if (diag) {
+        cs[2 * x] = c;
+        cs[2 * x + 1] = sn;
+      }
+      wave_sync();
+      const double c2 = cs[2 * x], s2 = cs[2 * x + 1];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {  // tile (4 g + u, x) <- R_j^T tile R_x
+        const double c1 = cs[2 * (4 * g + u)], s1 = cs[2 * (4 * g + u) + 1];
+        const double lpp = c1 * aii[u] - s1 * bji[u], lpq = c1 * bij[u] - s1 * bjj[u];
+        const double lqp = s1 * aii[u] + c1 * bji[u], lqq = s1 * bij[u] + c1 * bjj[u];
+        aii[u] = c2 * lpp - s2 * lpq;
+        bij[u] = s2 * lpp + c2 * lpq;
+        bji[u] = c2 * lqp - s2 * lqq;
+        bjj[u] = s2 * lqp + c2 * lqq;
+      }
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {  // Q <- Q J: columns I_x and J_sigma(x)
+        const double a = qi[v], b = qj[v];
+        qi[v] = c2 * a - s2 * b;
+        qj[v] = s2 * a + c2 * b;
+      }
+      wave_sync();  // the coefficients are read before the next round writes them
+    }
+    // advance sigma: J-side columns take slot x + 1, J-side rows slot j + 1
+#pragma unroll
+    for (int u = 0; u < 4; ++u) bij[u] = __shfl(bij[u], nxt, 64);
+#pragma unroll
+    for (int v = 0; v < 8; ++v) qj[v] = __shfl(qj[v], nxt, 64);
+    const double tji = __shfl(bji[0], down, 64);
+    const double tjj = __shfl(bjj[0], down_nxt, 64);
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      bji[u] = bji[u + 1];
+      bjj[u] = __shfl(bjj[u + 1], nxt, 64);
+    }
+    bji[3] = tji;
+    bjj[3] = tjj;
+  }
+  if (any) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = 4 * g + u;
+      const int gij = bi * BS + j, gjj = bj * BS + j;
+      if (gij < n && gix < n) M.A[(size_t)gij * nmax + gix] = aii[u];
+      if (gij < n && gjx < n) M.A[(size_t)gij * nmax + gjx] = bij[u];
+      if (gjj < n && gix < n) M.A[(size_t)gjj * nmax + gix] = bji[u];
+      if (gjj < n && gjx < n) M.A[(size_t)gjj * nmax + gjx] = bjj[u];
+    }
+    double* Q = M.Qw + (size_t)k * SB * SB;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const int i = 8 * g + v;
+      Q[i * SB + x] = qi[v];
+      Q[i * SB + BS + x] = qj[v];
+    }
+  }
+  wave_sync();
+  return any;
+}
+
 // Inner phase of block pair k at outer step s, by one wavefront: stage the
 // 32x32 subproblem, sweep it (all pairs at step 0, where every block has a
 // partner; the pairs across the two blocks afterwards), write the diagonal
 // tile and the rotation back.  Returns whether anything was rotated.
 __device__ bool bj_inner_pair(WaveLds& W, const BjMat& M, int k, int s, double thr, int lane) {
+  if (s > 0) return bj_inner_cross(W, M, k, s, thr, lane);
   int bi, bj;
   block_pair(k, s, M.nb, bi, bj);
   double v[16];

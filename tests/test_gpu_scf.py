@@ -197,14 +197,18 @@ def test_forces_central_differences_config2(cuda):
     rng = np.random.default_rng(2026)
     picks = rng.choice(200, size=24, replace=False)
     h = 1e-4
-    # base geometries first: keep molecules that converge well inside the
-    # iteration limit (small-gap molecules are ill-conditioned: displaced
-    # copies of one that converges in 182 iterations need 210 to over 250)
+    # small-gap molecules are ill-conditioned: a displaced copy of one that
+    # converges in 182 iterations can need over 250, so this test allows 500
+    # (the fixed point does not depend on the limit) and keeps the slow
+    # molecules that set the batch's tail
+    opts = hscf.SCFOptions(maxiter=500)
     N, P = workloads.pad([nums[i] for i in picks], [poss[i] for i in picks])
     bb = hbasis.build_batch(N, "gfn2", cuda)
-    res = hscf.run_scf(hscf.GeometryState(bb, torch.as_tensor(P, device=cuda)))
-    ok = [int(i) for i, c, it in zip(picks, res.converged, res.iterations) if c and it <= 120][:16]
+    res = hscf.run_scf(hscf.GeometryState(bb, torch.as_tensor(P, device=cuda), opts=opts))
+    ok = [int(i) for i, c in zip(picks, res.converged) if c][:16]
     assert len(ok) == 16
+    print("case config-2 FD: base iterations", sorted(int(it) for i, it in zip(picks, res.iterations) if int(i) in ok),
+          flush=True)
     F = hscf.forces(res).cpu()
     items, geoms, zs = [], [], []
     for i in ok:
@@ -221,8 +225,8 @@ def test_forces_central_differences_config2(cuda):
           flush=True)
     N2, P2 = workloads.pad(zs, geoms)
     bb2 = hbasis.build_batch(N2, "gfn2", cuda)
-    res2 = hscf.run_scf(hscf.GeometryState(bb2, torch.as_tensor(P2, device=cuda)))
-    assert bool(res2.converged.all())
+    res2 = hscf.run_scf(hscf.GeometryState(bb2, torch.as_tensor(P2, device=cuda), opts=opts))
+    assert bool(res2.converged.all()), np.sort(res2.iterations)[-8:]
     e = res2.energy.cpu().numpy()
     row = {int(i): r for r, i in enumerate(picks)}
     worst = 0.0
