@@ -48,7 +48,7 @@ constexpr int NWAVE = JB / 64;         // its wavefronts
 constexpr int JBB = 512;               // threads per workgroup of the blocked variant
 constexpr int NWB = JBB / 64;          // its wavefronts (one block pair or product item each)
 constexpr int JMAX_LDS = 128;          // largest matrix of the scalar variant
-constexpr int JMAX = 1536;             // largest matrix overall
+constexpr int JMAX = 4096;             // largest matrix overall
 constexpr int JSCALAR_AUTO = 64;       // auto: scalar variant up to this order
 constexpr int BS = 16;                 // block size of the blocked variant
 constexpr int SB = 2 * BS;             // subproblem order

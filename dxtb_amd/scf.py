@@ -3,7 +3,7 @@ Batched GFN1/GFN2-xTB self-consistent field on the GPU.
 
 Per geometry (once):  CN, H0 pair factors, repulsion, AES2 radii
 (``dxtb_setup``); S, D, Q and H0 (``dxtb_multipole_fwd``); the orthogonaliser
-X = L^-T of the Cholesky factor of S.
+X = V diag(l)^-1/2 from the Jacobi eigendecomposition of S.
 
 Per SCF iteration, for all active molecules of the batch:
   Fock build (``dxtb_fock``) -> F' = X^T F X -> eigensolve -> C = X C'
@@ -68,7 +68,7 @@ class SCFOptions:
     maxiter: int = 250
     damp: float = 0.5
     alpha: float = -0.5
-    eigensolver: str = "jacobi"  # "jacobi": hand-written Jacobi for nao <= 1536, else device eigh
+    eigensolver: str = "jacobi"  # "jacobi": hand-written Jacobi for nao <= 4096, else device eigh
     # Jacobi bar tied to the SCF residual (jacobi_tolerance).  Off: on the
     # config-2 molecules (small gaps) the inexact early solves cost more
     # Broyden iterations than they save in sweeps (750 instead of 1011 of 1024
@@ -124,8 +124,10 @@ class _SolveHistory:
     HISTORY = 8
 
     def __init__(self, n: int = 0):
-        self.margin = 1 if n <= 512 else 3
-        self.fresh_sweeps = self.FRESH + self.margin - 1
+        # a missing split sweep costs a one-workgroup sweep: 20-140 ms above
+        # order 512, seconds above 1536
+        self.margin = 1 if n <= 512 else (3 if n <= 1536 else 4)
+        self.fresh_sweeps = self.FRESH + self.margin - 1 + (6 if n > 1536 else 0)
         self.recent: dict[int, list[int]] = {}  # size class -> last counts
         self.bump = False
         self.calls = 0
@@ -176,17 +178,37 @@ class _SolveHistory:
         self.ev_class = self.size_class(nb)
 
 
-def _orthogonaliser(spad: torch.Tensor, chunk: int = 32) -> torch.Tensor:
-    """X = L^-T with S = L L^T (so X^T S X = 1), in chunks of matrices: the
-    device batched triangular solve fails for large batch counts."""
+# eigensolves of the overlap matrix (orthogonaliser): from the unit matrix,
+# to the rounding floor (off-diagonal 1e-15), sweeps in the split form
+S_JACOBI_TOL = 1.0e-15
+S_SPLIT_SWEEPS = 16
+
+
+def _orthogonaliser(spad: torch.Tensor, nsize: torch.Tensor, variant: int = 3) -> torch.Tensor:
+    """X with X^T S X = 1 from the eigendecomposition S = V diag(l) V^T by the
+    hand-written Jacobi solver (canonical orthogonalisation X = V diag(l)^-1/2).
+    The reference orthogonalises with the Cholesky factor (X = L^-T,
+    exlibs/xitorch/_impls/linalg/symeig.py:38-75); any X with X^T S X = 1
+    spans the same generalized eigenproblem, so orbitals, density and
+    energies are the same (to rounding; degenerate orbitals may mix).
+    Padding rows/columns of S carry 1 on the diagonal and keep X = 1 there."""
+    lib = _lib.load()
     B, n, _ = spad.shape
-    X = torch.empty_like(spad)
-    eye = torch.eye(n, dtype=spad.dtype, device=spad.device)
-    for i in range(0, B, chunk):
-        L = torch.linalg.cholesky(spad[i : i + chunk])
-        Y = torch.linalg.solve_triangular(L, eye.expand(L.shape[0], n, n), upper=False)
-        X[i : i + chunk] = Y.transpose(-1, -2)
-    return X
+    dev = spad.device
+    A = spad.clone()
+    VT = torch.empty_like(A)
+    ev = torch.empty((B, n), dtype=torch.float64, device=dev)
+    work = torch.empty(max(lib.dxtb_jacobi_work_size(B, n) // 8, 1), dtype=torch.float64, device=dev)
+    var = 1 if n <= JACOBI_SCALAR_MAX else variant
+    split = S_SPLIT_SWEEPS + (6 if n > 1536 else 0)
+    max_sweeps = 60 if n <= 1536 else split  # one-workgroup sweeps take seconds above order 1536
+    rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(nsize), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work), var,
+                              max_sweeps, split, S_JACOBI_TOL, None, None, _lib.stream_ptr(dev))
+    _lib.check(rc, "dxtb_jacobi_eigh (overlap)")
+    live = torch.arange(n, device=dev)[None, :] < nsize[:, None]
+    scale = torch.where(live, ev.clamp(min=torch.finfo(torch.float64).tiny).rsqrt(), 1.0)
+    # X[i, j] = V[i, j] / sqrt(l_j) = VT[j, i] / sqrt(l_j)
+    return (VT.transpose(1, 2) * scale.unsqueeze(1)).contiguous()
 
 
 def alpha_beta(nel: np.ndarray, uhf=None) -> np.ndarray:
@@ -258,11 +280,16 @@ class GeometryState:
         _lib.check(rc, "dxtb_setup")
         self.S, self.D, self.Q, self.H0 = multipole_forward(bb, self.pos, self.mp, fsh=self.fsh)
         self._setup_solvation(solvation, st)
-        # orthogonaliser X = L^-T with unit diagonal on padding
+        # orthogonaliser (X^T S X = 1) with unit diagonal on padding
         spad = self.S.clone()
         pad = torch.arange(n, device=dev)[None, :] >= d["nao"][:, None]
         spad.diagonal(dim1=-2, dim2=-1).copy_(torch.where(pad, 1.0, spad.diagonal(dim1=-2, dim2=-1)))
-        self.X = _orthogonaliser(spad)
+        if n <= self.lib.dxtb_jacobi_max_n():
+            self.X = _orthogonaliser(spad, d["nao"], self.opts.jacobi_variant)
+        else:
+            L = torch.linalg.cholesky(spad)
+            eye = torch.eye(n, dtype=spad.dtype, device=dev)
+            self.X = torch.linalg.solve_triangular(L, eye.expand(B, n, n), upper=False).transpose(-1, -2).contiguous()
         # reference occupations and electron counts (scf/iterator.py:147-195)
         tab_occ = _refocc_table(bb.gfn)
         n0 = np.zeros((B, n))
@@ -486,7 +513,7 @@ class GeometryState:
     def solve(self, F):
         """Orbitals, occupations and density of a Fock matrix.
 
-        nao <= 1536: the hand-written Jacobi solver on A = C^T F C with
+        nao <= 4096: the hand-written Jacobi solver on A = C^T F C with
         C = X U the previous iteration's orbitals (S-orthonormal, so A is the
         Fock matrix in the old orbital basis and nearly diagonal; the first
         iteration uses U = 1, C = X = L^-T); the new orbitals are C V and U
@@ -518,8 +545,10 @@ class GeometryState:
             if prof is not None:
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 ev[0].record(torch.cuda.current_stream(self.dev))
+            # above order 1536 a one-workgroup sweep takes seconds: every sweep runs split
+            max_sweeps = 50 if n <= 1536 else split
             rc = self.lib.dxtb_jacobi_eigh(nb, n, _lib.ptr(nsz), _lib.ptr(A), _lib.ptr(V),
-                                           _lib.ptr(e), _lib.ptr(self._jwork), variant, 50, split, JACOBI_TOL,
+                                           _lib.ptr(e), _lib.ptr(self._jwork), variant, max_sweeps, split, JACOBI_TOL,
                                            _lib.ptr(self._jtol), _lib.ptr(sw), _lib.stream_ptr(self.dev))
             _lib.check(rc, "dxtb_jacobi_eigh")
             self._jhist.record(nb, self.dev)

@@ -360,7 +360,7 @@ class Engine:
     opts: hscf.SCFOptions = field(default_factory=hscf.SCFOptions)
     capacity: dict | None = None  # class nao -> slots (default: default_capacity)
     # one captured hipGraph per pool iteration (pools within the Jacobi
-    # solver's order limit, 1536): cuts the host work per iteration, but the
+    # solver's order limit): cuts the host work per iteration, but the
     # static windows and the full Broyden history cost more device time than
     # that saves on the measured workloads
     graphs: bool = False
@@ -394,20 +394,17 @@ class Engine:
 
     def _admit_into(self, c: int, p: "SlotPool") -> None:
         """Admit waiting requests of class ``c`` into the free slots of pool
-        ``p`` (no iteration of the pool in flight).  The setup runs on the
-        default stream (it uses the vendor solvers, Cholesky and triangular
-        solve, which are not reliably ordered with a side stream); the pool's
-        stream then waits for it."""
+        ``p`` (no iteration of the pool in flight).  The set-up (integrals,
+        H0, orthogonaliser by the Jacobi solver, solvation) runs on the pool's
+        own stream after the copies of retiring rows, so other pools are not
+        held up."""
         q = self.queue.get(c)
         if not q or p.pending is not None:
             return
         m = min(p.nfree, len(q))
         if m:
-            dflt = torch.cuda.default_stream(torch.device(self.device))
-            dflt.wait_stream(p.stream)  # copies of retiring rows are taken before the rows are refilled
-            with torch.cuda.stream(dflt):
+            with torch.cuda.stream(p.stream):
                 p.admit(q[:m], self.now())
-            p.stream.wait_stream(dflt)
             del q[:m]
 
     def admit(self) -> None:

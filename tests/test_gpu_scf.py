@@ -240,10 +240,10 @@ def test_forces_central_differences_config2(cuda):
 
 def test_water_cluster_config4(cuda):
     """Config 4 at full size: 667 rigid waters (2001 atoms, 4002 basis
-    functions, past the Jacobi solver's order limit, so the device eigh
-    runs).  The SCF converges, the forces sum to zero (translation
-    invariance), and two force components match central differences of the
-    energy (h = 1e-4 Bohr)."""
+    functions; the hand-written Jacobi solver in its split form for the
+    orthogonaliser and every SCF iteration, no vendor solver).  The SCF
+    converges, the forces sum to zero (translation invariance), and two force
+    components match central differences of the energy (h = 1e-4 Bohr)."""
     from dxtb_amd import workloads
 
     z, p = workloads.water_cluster()
