@@ -637,8 +637,73 @@ def cpu_baseline_gfn2(nums, poss, chg, label, solv=None, budget_s=20.0):
     }
 
 
+EIGEN_KERNELS = ("k_bj_", "k_jacobi_")
+
+
+def trace_summary(path: str, steps: int = 1) -> None:
+    """Summary of a rocprofv3 kernel trace (``--kernel-trace --output-format
+    csv``, the ``*_kernel_trace.csv`` file): per-kernel calls, total and mean
+    duration, the device busy time (union of all kernel intervals) and the
+    eigensolver's busy time (union of its kernels' intervals), per step of the
+    traced run; no GPU needed."""
+    import csv
+
+    rows = []
+    with open(path) as fh:
+        for d in csv.DictReader(fh):
+            rows.append((int(d["Start_Timestamp"]), int(d["End_Timestamp"]),
+                         d["Kernel_Name"].replace("(anonymous namespace)::", "")))
+    if not rows:
+        print("no kernels")
+        return
+    tot: dict = {}
+    for a, b, name in rows:
+        k = name.split("(")[0].replace("void ", "")[:80]
+        c, t = tot.get(k, (0, 0))
+        tot[k] = (c + 1, t + b - a)
+    span = (max(b for _, b, _ in rows) - min(a for a, _, _ in rows)) * 1e-6
+    busy = _union_ms([(a * 1e-6, b * 1e-6) for a, b, _ in rows])
+    eig = [(a * 1e-6, b * 1e-6) for a, b, n in rows if any(k in n for k in EIGEN_KERNELS)]
+    eig_busy = _union_ms(eig)
+    allsum = sum(t for _, t in tot.values()) * 1e-6
+    print(f"kernels {len(rows)}; trace span {span:.1f} ms; device busy (union) {busy:.1f} ms; "
+          f"summed kernel time {allsum:.1f} ms; eigensolver busy (union of its kernels) {eig_busy:.1f} ms "
+          f"({100 * eig_busy / busy:.1f} % of busy); per step ({steps}): busy {busy / steps:.1f} ms, "
+          f"eigensolver {eig_busy / steps:.1f} ms")
+    print("| kernel | calls | total ms | % of summed | mean us |")
+    print("|---|---:|---:|---:|---:|")
+    for k, (c, t) in sorted(tot.items(), key=lambda kv: -kv[1][1])[:30]:
+        print(f"| `{k}` | {c} | {t * 1e-6:.1f} | {100 * t * 1e-6 / allsum:.1f} | {t * 1e-3 / c:.1f} |")
+
+
+def pmc_summary(path: str) -> None:
+    """Summary of a rocprofv3 counter collection (``--pmc ... --output-format
+    csv``, the ``*_counter_collection.csv`` file): per kernel and counter, the
+    dispatches and the counter's sum and mean per dispatch; no GPU needed."""
+    import csv
+
+    acc: dict = {}
+    with open(path) as fh:
+        for d in csv.DictReader(fh):
+            k = d["Kernel_Name"].replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")[:70]
+            key = (k, d["Counter_Name"])
+            n, v, ids = acc.get(key, (0, 0.0, set()))
+            ids.add(d.get("Dispatch_Id", ""))
+            acc[key] = (n + 1, v + float(d["Counter_Value"]), ids)
+    print("| kernel | counter | dispatches | sum | mean per dispatch |")
+    print("|---|---|---:|---:|---:|")
+    for (k, cn), (n, v, ids) in sorted(acc.items(), key=lambda kv: -kv[1][1]):
+        nd = max(len(ids), 1)
+        print(f"| `{k}` | {cn} | {nd} | {v:.0f} | {v / nd:.1f} |")
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--pmc-summary", default=None, metavar="COUNTER_COLLECTION_CSV",
+                    help="summarise a rocprofv3 counter collection instead of benchmarking (no GPU)")
+    ap.add_argument("--trace-summary", default=None, metavar="KERNEL_TRACE_CSV",
+                    help="summarise a rocprofv3 kernel trace instead of benchmarking (no GPU)")
+    ap.add_argument("--trace-steps", type=int, default=1)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
@@ -655,6 +720,12 @@ def main():
     ap.add_argument("--jacobi-variant", type=int, default=3, choices=[0, 2, 3],
                     help="eigensolver form above order 64: 3 split (default), 2 one workgroup per matrix, 0 auto")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(args.trace_summary, args.trace_steps)
+        return
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only (one process, one device); --gpus must be 1"}))
         sys.exit(2)

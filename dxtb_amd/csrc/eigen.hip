@@ -262,8 +262,7 @@ constexpr int PROD_TILE = SB * SB;  // doubles per staging tile
 constexpr size_t PROD_LDS = (size_t)PW * PROD_TILE * sizeof(double);
 
 struct BlockedLds {
-  WaveLds w[NWB];
-  int rot[KMAX];  // pair k rotated anything in this outer step
+  WaveLds w[NWB];  // reused between sweeps as the VT panel of the deferred update
   double red[NWB];
 };
 
@@ -444,27 +443,61 @@ __device__ __forceinline__ void block_pair(int k, int s, int nb, int& bi, int& b
   bj = max(a, c);
 }
 
+// Deferred eigenvector update: up to this order the rotations of a whole
+// outer sweep are kept (a log of (nb - 1) steps x npairs 32x32 blocks per
+// matrix) and applied to VT once per sweep, a 32-column panel of all rows at
+// a time in LDS (k_bj_apply, or the one-workgroup kernel between sweeps);
+// above it every step updates VT panels in its product phase.
+constexpr int DEFER_MAX = 512;
+// ... and for batches of more than this many matrices (batch capacity): a
+// lone matrix is bound by the latency of the sweep's launch chain, which the
+// extra, MFMA-bound panel update lengthens, while full batches gain from
+// reading and writing VT once per sweep instead of once per step
+constexpr int DEFER_MIN_BATCH = 16;
+
+__host__ __device__ constexpr bool defer_vt(int nmax) { return nmax <= DEFER_MAX; }
+__host__ __device__ constexpr bool defer_call(int nmax, int nbcap) {
+  return defer_vt(nmax) && nbcap > DEFER_MIN_BATCH;
+}
+// steps of rotations kept per matrix: a sweep's worth when deferred, else one
+__host__ __device__ constexpr int log_steps(int nmax) {
+  return defer_vt(nmax) ? 2 * ((nmax + SB - 1) / SB) - 1 : 1;
+}
+
 // One matrix of the blocked variant.
 struct BjMat {
-  double* A;   // (nmax, nmax), in place
-  double* V;   // eigenvector rows VT
-  double* Qw;  // rotations of the current outer step, npairs x 32 x 32
-  int n, nmax, nlast, npairs, nb;
+  double* A;      // (nmax, nmax), in place
+  double* V;      // eigenvector rows VT
+  double* Qlog;   // rotations: log_steps x npad x 32 doubles (npairs 32x32 blocks per step)
+  int* flags;     // per step and block pair: rotated anything (log_steps x KMAX)
+  int n, nmax, nlast, npairs, nb, npad;
+  bool defer;
 };
 
-__device__ __forceinline__ BjMat bj_mat(int b, int nmax, const int* nsize, double* Ag, double* VT, double* work) {
+__device__ __forceinline__ BjMat bj_mat(int b, int nmax, const int* nsize, double* Ag, double* VT, double* work,
+                                       int* flags, int defer) {
   BjMat M;
   const int npad = (nmax + SB - 1) / SB * SB;
+  const int nlog = log_steps(nmax);
   M.n = min(max(nsize[b], 0), nmax);
   M.nmax = nmax;
   M.nlast = max(M.n - 1, 0);
   M.npairs = (M.n + SB - 1) / SB;  // block pairs (n padded to 32)
   M.nb = 2 * M.npairs;             // blocks (even)
+  M.npad = npad;
+  M.defer = defer_vt(nmax) && defer != 0;
   M.A = Ag + (size_t)b * nmax * nmax;
   M.V = VT + (size_t)b * nmax * nmax;
-  M.Qw = work + (size_t)b * npad * SB;
+  M.Qlog = work + (size_t)b * nlog * npad * SB;
+  M.flags = flags + (size_t)b * nlog * KMAX;
   return M;
 }
+
+// rotations and flags of outer step s
+__device__ __forceinline__ double* qstep(const BjMat& M, int s) {
+  return M.Qlog + (size_t)(M.defer ? s : 0) * M.npad * SB;
+}
+__device__ __forceinline__ int* fstep(const BjMat& M, int s) { return M.flags + (M.defer ? s : 0) * KMAX; }
 
 // Data pointers and batch size of one dxtb_jacobi_eigh call, written into the
 // head of the workspace by k_bj_args before the call's other kernels run.
@@ -481,7 +514,7 @@ struct JArgs {
   const double* tol_b;
   int* sweeps;
   int nbatch;
-  int pad;
+  int defer;  // deferred eigenvector update (defer_call of the call's sizes)
 };
 constexpr size_t JARGS_BYTES = 256;  // head of the workspace (keeps the rest aligned)
 
@@ -605,7 +638,7 @@ __device__ bool bj_inner_cross(WaveLds& W, const BjMat& M, int k, int s, double 
       if (gjj < n && gix < n) M.A[(size_t)gjj * nmax + gix] = bji[u];
       if (gjj < n && gjx < n) M.A[(size_t)gjj * nmax + gjx] = bjj[u];
     }
-    double* Q = M.Qw + (size_t)k * SB * SB;
+    double* Q = qstep(M, s) + (size_t)k * SB * SB;
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
       const int i = 8 * g + v;
@@ -649,7 +682,7 @@ __device__ bool bj_inner_pair(WaveLds& W, const BjMat& M, int k, int s, double t
       const int r = e >> 5, c = e & 31;
       const int gr = gidx(r, bi, bj), gc = gidx(c, bi, bj);
       if (gr < M.n && gc < M.n) M.A[(size_t)gr * M.nmax + gc] = W.S[r * SST + c];
-      M.Qw[(size_t)k * SB * SB + e] = W.Q[r * SST + c];
+      qstep(M, s)[(size_t)k * SB * SB + e] = W.Q[r * SST + c];
     }
   }
   wave_sync();
@@ -694,7 +727,7 @@ __device__ void bj_product_item(double* __restrict__ Sx, const BjMat& M, const i
     };
     d4_t acc[2][2];
     if (rk) {
-      qt_times(M.Qw + (size_t)k * SB * SB, loadA, lane, acc);
+      qt_times(qstep(M, s) + (size_t)k * SB * SB, loadA, lane, acc);
     } else {  // Q_k = 1: T is the tile itself
 #pragma unroll
       for (int rb = 0; rb < 2; ++rb)
@@ -715,7 +748,7 @@ __device__ void bj_product_item(double* __restrict__ Sx, const BjMat& M, const i
     if (rk2) {
       // U^T = Q_k2^T T^T with U = T Q_k2: the same product, T read transposed
       auto loadT = [&](int m, int c) -> double { return Sx[tix<SWZ>(c, m)]; };
-      qt_times(M.Qw + (size_t)k2 * SB * SB, loadT, lane, acc);
+      qt_times(qstep(M, s) + (size_t)k2 * SB * SB, loadT, lane, acc);
 #pragma unroll
       for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -756,7 +789,7 @@ __device__ void bj_product_item(double* __restrict__ Sx, const BjMat& M, const i
       return x * mask01(gr < n && gc < n);
     };
     d4_t acc[2][2];
-    qt_times(M.Qw + (size_t)k * SB * SB, loadV, lane, acc);
+    qt_times(qstep(M, s) + (size_t)k * SB * SB, loadV, lane, acc);
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -769,8 +802,54 @@ __device__ void bj_product_item(double* __restrict__ Sx, const BjMat& M, const i
   }
 }
 
+// product items of a step: the off-diagonal tiles, and the VT panels unless
+// the eigenvector update is deferred to the end of the sweep
 __device__ __forceinline__ int bj_nwork(const BjMat& M) {
-  return M.npairs * (M.npairs - 1) / 2 + M.npairs * ((M.n + SB - 1) / SB);
+  return M.npairs * (M.npairs - 1) / 2 + (M.defer ? 0 : M.npairs * ((M.n + SB - 1) / SB));
+}
+
+// Deferred eigenvector update of the 32-column panel pc of VT: every step s of
+// the sweep replaces the rows IJ_k of the panel by Q_k^T (rows IJ_k) for the
+// block pairs that rotated, on the matrix cores, with the panel (all rows) in
+// LDS at P (npad x PST doubles).  Executed by all nw wavefronts of the
+// workgroup (barriers inside).
+constexpr int PST = SB + 1;  // LDS row stride of a panel
+static_assert((size_t)DEFER_MAX * PST * sizeof(double) <= sizeof(BlockedLds),
+              "the deferred VT panel must fit the one-workgroup kernel's LDS");
+
+__device__ void bj_apply_panel(const BjMat& M, int pc, double* P, int wv, int nw, int lane) {
+  const int n = M.n, nmax = M.nmax, c0 = pc * SB;
+  const int nthr = nw * 64, tid = wv * 64 + lane;
+  for (int e = tid; e < M.npad * SB; e += nthr) {
+    const int r = e >> 5, c = e & 31;
+    const int rr = min(r, M.nlast), cc = min(c0 + c, M.nlast);
+    P[r * PST + c] = M.V[(size_t)rr * nmax + cc] * mask01(r < n && c0 + c < n);
+  }
+  __syncthreads();
+  for (int s = 0; s < M.nb - 1; ++s) {
+    const int* fl = fstep(M, s);
+    for (int k = wv; k < M.npairs; k += nw) {
+      if (!fl[k]) continue;
+      int bi, bj;
+      block_pair(k, s, M.nb, bi, bj);
+      auto loadP = [&](int m, int c) -> double { return P[gidx(m, bi, bj) * PST + c]; };
+      d4_t acc[2][2];
+      qt_times(qstep(M, s) + (size_t)k * SB * SB, loadP, lane, acc);
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            P[gidx(rb * BS + acc_row(lane, i), bi, bj) * PST + cb * BS + (lane & 15)] = acc[rb][cb][i];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * SB; e += nthr) {
+    const int r = e >> 5, c = e & 31;
+    if (c0 + c < n) M.V[(size_t)r * nmax + c0 + c] = P[r * PST + c];
+  }
+  __syncthreads();
 }
 
 // max |offdiag| and max |diag| of the matrix, by the whole workgroup
@@ -804,8 +883,9 @@ __device__ void bj_norms(const BjMat& M, double* red, double& off, double& amax)
 // the split kernels left; sweep_base is then their per-matrix state, whose
 // fourth entry counts the sweeps already done.
 __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const JArgs* __restrict__ ja,
-                                                       double* __restrict__ work, int max_sweeps, double tol,
-                                                       int init_v, const double* __restrict__ sweep_base) {
+                                                       double* __restrict__ work, int* __restrict__ flags,
+                                                       int max_sweeps, double tol, int init_v,
+                                                       const double* __restrict__ sweep_base) {
   extern __shared__ double lds_raw[];
   BlockedLds& L = *reinterpret_cast<BlockedLds*>(lds_raw);
   const int b = blockIdx.x;
@@ -814,7 +894,7 @@ __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const JArgs* _
   const double* __restrict__ tol_b = jc->tol_b;
   int* __restrict__ sweeps_out = jc->sweeps;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work);
+  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, flags, jc->defer);
   double* eb = jc->evals + (size_t)b * nmax;
   const int base = sweep_base ? (int)sweep_base[4 * b + 3] : 0;
 
@@ -842,12 +922,15 @@ __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const JArgs* _
     for (int s = 0; s < M.nb - 1; ++s) {
       for (int k = wv; k < M.npairs; k += NWB) {  // inner phase: one wavefront per block pair
         const bool any = bj_inner_pair(W, M, k, s, thr, lane);
-        if (lane == 0) L.rot[k] = any ? 1 : 0;
+        if (lane == 0) fstep(M, s)[k] = any ? 1 : 0;
       }
       __syncthreads();
-      const int nwork = bj_nwork(M);  // product phase: tiles and VT panels
-      for (int item = wv; item < nwork; item += NWB) bj_product_item<false>(W.S, M, L.rot, item, s, lane);
+      const int nwork = bj_nwork(M);  // product phase: tiles (and VT panels unless deferred)
+      for (int item = wv; item < nwork; item += NWB) bj_product_item<false>(W.S, M, fstep(M, s), item, s, lane);
       __syncthreads();
+    }
+    if (M.defer) {  // the sweep's rotations onto VT, panel by panel (LDS reused)
+      for (int pc = 0; pc < M.npairs; ++pc) bj_apply_panel(M, pc, lds_raw, wv, NWB, lane);
     }
   }
   for (int i = threadIdx.x; i < nmax; i += JBB) eb[i] = (i < M.n) ? M.A[(size_t)i * nmax + i] : 0.0;
@@ -884,7 +967,7 @@ __global__ __launch_bounds__(JBB) void k_bj_check(int nmax, const JArgs* __restr
   const JArgsC jc = jconst(ja);
   if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
   const double* __restrict__ tol_b = jc->tol_b;
-  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work);
+  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, nullptr, 0);
   double off, amax;
   bj_norms(M, red, off, amax);
   const double thr = fmax(1e-300, (tol_b ? tol_b[b] : tol) * fmax(amax, 1.0));
@@ -905,24 +988,40 @@ __global__ __launch_bounds__(64) void k_bj_inner(int nmax, const JArgs* __restri
   const int b = blockIdx.y, k = blockIdx.x;
   const JArgsC jc = jconst(ja);
   if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
-  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work);
+  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, rot, jc->defer);
   if (k >= M.npairs || s >= M.nb - 1) return;
   const bool any = bj_inner_pair(W, M, k, s, sst[4 * b + 1], threadIdx.x);
-  if (threadIdx.x == 0) rot[b * KMAX + k] = any ? 1 : 0;
+  if (threadIdx.x == 0) fstep(M, s)[k] = any ? 1 : 0;
 }
 
 __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(5))) void k_bj_product(int nmax, const JArgs* __restrict__ ja, double* __restrict__ work,
-                                                   const double* __restrict__ sst, const int* __restrict__ rot,
+                                                   const double* __restrict__ sst, int* __restrict__ rot,
                                                    int s) {
   extern __shared__ double lds_raw[];  // one staging tile per wavefront
   const int b = blockIdx.y;
   const JArgsC jc = jconst(ja);
   if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
-  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work);
+  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, rot, jc->defer);
   if (s >= M.nb - 1) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int item = blockIdx.x * PW + wv;
-  if (item < bj_nwork(M)) bj_product_item<true>(lds_raw + (size_t)wv * PROD_TILE, M, rot + b * KMAX, item, s, lane);
+  if (item < bj_nwork(M)) bj_product_item<true>(lds_raw + (size_t)wv * PROD_TILE, M, fstep(M, s), item, s, lane);
+}
+
+// Deferred eigenvector update of one sweep: one workgroup per (matrix, VT
+// panel of 32 columns), the panel's rows in LDS.
+constexpr int AW = 4;  // wavefronts per apply workgroup
+
+__global__ __launch_bounds__(AW * 64) void k_bj_apply(int nmax, const JArgs* __restrict__ ja,
+                                                     double* __restrict__ work, const double* __restrict__ sst,
+                                                     int* __restrict__ rot) {
+  extern __shared__ double lds_raw[];
+  const int b = blockIdx.y, pc = blockIdx.x;
+  const JArgsC jc = jconst(ja);
+  if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
+  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, rot, jc->defer);
+  if (pc >= M.npairs) return;  // n <= 32 npairs: one panel per block pair
+  bj_apply_panel(M, pc, lds_raw, threadIdx.x >> 6, AW, threadIdx.x & 63);
 }
 
 bool attr_done[64];
@@ -941,12 +1040,15 @@ int batch_capacity(int nbatch) {
   return (nbatch + 31) / 32 * 32;
 }
 
-// workspace: argument block (JARGS_BYTES), rotations (nbcap x npad x 32
-// doubles), split state (nbcap x 4 doubles), rotation flags (nbcap x KMAX ints)
+// workspace: argument block (JARGS_BYTES), rotations (nbcap x log_steps x
+// npad x 32 doubles), split state (nbcap x 4 doubles), rotation flags (nbcap x
+// log_steps x KMAX ints)
 size_t rot_bytes(int nbcap, int nmax) {
   const size_t npad = (size_t)(nmax + SB - 1) / SB * SB;
-  return (size_t)nbcap * npad * SB * sizeof(double);
+  return (size_t)nbcap * log_steps(nmax) * npad * SB * sizeof(double);
 }
+
+size_t apply_lds(int nmax) { return (size_t)((nmax + SB - 1) / SB * SB) * PST * sizeof(double); }
 
 struct Layout {
   JArgs* ja;
@@ -970,7 +1072,8 @@ Layout layout(double* work, int nbcap, int nmax) {
 // Every argument is fixed by (work, nbcap, nmax, nsw, max_sweeps, tol).
 hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax, int nsw, int max_sweeps, double tol) {
   const int npairs = (nmax + SB - 1) / SB, nb = 2 * npairs;
-  const int nwork = npairs * (npairs - 1) / 2 + npairs * npairs;
+  const bool defer = defer_call(nmax, nbcap);
+  const int nwork = npairs * (npairs - 1) / 2 + (defer ? 0 : npairs * npairs);
   if (nsw > 0) {
     hipLaunchKernelGGL(k_bj_init, dim3((unsigned)nbcap), dim3(JBB), 0, st, nmax, w.ja, w.sst);
     for (int sw = 0; sw < nsw; ++sw) {
@@ -978,13 +1081,19 @@ hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax,
       for (int s = 0; s < nb - 1; ++s) {
         hipLaunchKernelGGL(k_bj_inner, dim3((unsigned)npairs, (unsigned)nbcap), dim3(64), 0, st, nmax, w.ja,
                            w.rot_q, w.sst, w.rot, s);
-        hipLaunchKernelGGL(k_bj_product, dim3((unsigned)((nwork + PW - 1) / PW), (unsigned)nbcap), dim3(PW * 64),
-                           PROD_LDS, st, nmax, w.ja, w.rot_q, w.sst, w.rot, s);
+        if (nwork > 0)  // one block pair with deferred VT: nothing to multiply
+          hipLaunchKernelGGL(k_bj_product, dim3((unsigned)((nwork + PW - 1) / PW), (unsigned)nbcap), dim3(PW * 64),
+                             PROD_LDS, st, nmax, w.ja, w.rot_q, w.sst, w.rot, s);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
       }
+      if (defer)
+        hipLaunchKernelGGL(k_bj_apply, dim3((unsigned)npairs, (unsigned)nbcap), dim3(AW * 64), apply_lds(nmax), st,
+                           nmax, w.ja, w.rot_q, w.sst, w.rot);
     }
   }
   hipLaunchKernelGGL(k_jacobi_blocked, dim3((unsigned)nbcap), dim3(JBB), sizeof(BlockedLds), st, nmax, w.ja, w.rot_q,
-                     max_sweeps - nsw, tol, nsw > 0 ? 0 : 1, nsw > 0 ? w.sst : nullptr);
+                     w.rot, max_sweeps - nsw, tol, nsw > 0 ? 0 : 1, nsw > 0 ? w.sst : nullptr);
   return hipGetLastError();
 }
 
@@ -1081,7 +1190,7 @@ extern "C" size_t dxtb_jacobi_work_size(int nbatch, int nmax) {
   if (nbatch <= 0 || nmax <= 0) return 0;
   const int nbcap = batch_capacity(nbatch);
   return JARGS_BYTES + rot_bytes(nbcap, nmax) + (size_t)nbcap * 4 * sizeof(double) +
-         (size_t)nbcap * KMAX * sizeof(int);
+         (size_t)nbcap * log_steps(nmax) * KMAX * sizeof(int);
 }
 
 extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals,
@@ -1128,6 +1237,8 @@ int jacobi_impl(int nbatch, int nmax, const int* nsize, double* A, double* VT, d
                                    (int)sizeof(BlockedLds)));
     DXTB_CHECK(hipFuncSetAttribute((const void*)k_bj_product, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)PROD_LDS));
+    DXTB_CHECK(hipFuncSetAttribute((const void*)k_bj_apply, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)apply_lds(DEFER_MAX)));
     attr_done[dev] = true;
   }
   if (scalar) {
@@ -1143,7 +1254,7 @@ int jacobi_impl(int nbatch, int nmax, const int* nsize, double* A, double* VT, d
   if (!split) nsw = 0;
   const int nbcap = batch_capacity(nbatch);
   const Layout w = layout(work, nbcap, nmax);
-  const JArgs args{nsize, A, VT, evals, tol_b, sweeps, nbatch, 0};
+  const JArgs args{nsize, A, VT, evals, tol_b, sweeps, nbatch, defer_call(nmax, nbcap) ? 1 : 0};
   hipLaunchKernelGGL(k_bj_args, dim3(1), dim3(64), 0, st, w.ja, args);
   DXTB_CHECK(hipGetLastError());
   // the recorded sequence needs a stream that can be captured: not the null

@@ -219,8 +219,8 @@ int dxtb_gb_forces(int nbatch, int nat_max, const int* nat, const int* numbers, 
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Arguments and results as dxtb_jacobi_eigh.  A call whose
  * workspace (dxtb_jacobi_work_size(nbatch, nmax)) exceeds the array returns
- * DXTB_ESCRATCH and launches nothing, e.g. beyond about 1000 matrices of
- * order 256 or 160 of order 1536.  All calls on one GPU share its array, so
+ * DXTB_ESCRATCH and launches nothing, e.g. beyond 64 matrices of order 256
+ * or 160 of order 1536.  All calls on one GPU share its array, so
  * calls that overlap on one GPU (different streams, or threads) race: order
  * them on one stream. */
 int dxtb_jacobi_eigh_noalloc(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals,

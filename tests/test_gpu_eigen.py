@@ -29,8 +29,11 @@ def _jacobi(A, nsize, variant=0, sweeps=60, tol=1e-14, tol_b=None, split=4):
     return e, V, sw
 
 
+# the last two batches exceed 16 matrices: the blocked forms then apply each
+# sweep's rotations to the eigenvectors once per sweep (deferred update)
 CASES = [(7, [7, 5, 1, 2]), (33, [33, 32, 17]), (64, [64, 63, 40]), (128, [128, 127, 100, 3]),
-         (300, [300, 211, 129]), (512, [512, 333]), (1030, [1030, 700])]
+         (300, [300, 211, 129]), (512, [512, 333]), (1030, [1030, 700]),
+         (96, [96 - 4 * k for k in range(20)]), (260, [260, 1] + [250 - 11 * k for k in range(18)])]
 
 
 @pytest.mark.parametrize("variant", [1, 2, 3])
@@ -95,7 +98,8 @@ def test_per_matrix_tolerance(cuda, variant):
     torch.testing.assert_close(e[1], e1[0], atol=0, rtol=0)
 
 
-@pytest.mark.parametrize("nmax,sizes", [(128, [128, 97]), (300, [300, 64, 0, 255]), (512, [512])])
+@pytest.mark.parametrize("nmax,sizes", [(128, [128, 97]), (300, [300, 64, 0, 255]), (512, [512]),
+                                        (200, [200, 0] + [199 - 9 * k for k in range(20)])])
 def test_split_matches_blocked(cuda, nmax, sizes):
     """The split form runs the same rotations as the one-workgroup form: the
     same sweep counts and eigenpairs to rounding (an empty matrix included)."""
