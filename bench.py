@@ -198,18 +198,39 @@ def jacobi_flops_per_sweep(n: int, variant: int) -> float:
 # measured HBM traffic of one dxtb_jacobi_eigh call / its algorithmic bytes,
 # by form: rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch (summed over the
 # call's kernels), scaled by the same counters on a copy of known size (eigen
-# workload, 256 x order 240, one sweep).  2: one workgroup per matrix; 3 (and
-# 0, which picks 3 or 2): the split form.  The scalar form (1, A in LDS) is
-# not measured and counted at its algorithmic bytes.
-JACOBI_TRAFFIC_RATIO = {0: 1.103, 1: 1.0, 2: 1.094, 3: 1.103}
+# workload, 256 x order 240, one sweep; FETCH_SIZE x 2, WRITE_SIZE x 1).
+# 2: one workgroup per matrix; 3 (and 0, which picks 3 or 2): the split form;
+# both with the deferred eigenvector update there (256 matrices).  The scalar
+# form (1, A in LDS) is not measured and counted at its algorithmic bytes.
+JACOBI_TRAFFIC_RATIO = {0: 1.103, 1: 1.0, 2: 1.154, 3: 1.103}
+# the library's choice of the deferred eigenvector update (eigen.hip:
+# defer_call): order <= 512 and a batch capacity above 16
+DEFER_MAX, DEFER_MIN_BATCH = 512, 16
 
 
-def jacobi_bytes(n: int, variant: int) -> float:
-    """Algorithmic HBM bytes of one sweep: blocked variant, A and VT read and
-    written once per outer step; scalar variant (A in LDS), the two VT rows of
-    every rotation read and written once per step."""
+def batch_capacity(nbatch: int) -> int:
+    """The eigensolver's batch capacity (eigen.hip: batch_capacity)."""
+    if nbatch <= 8:
+        c = 1
+        while c < nbatch:
+            c <<= 1
+        return c
+    if nbatch <= 64:
+        return (nbatch + 7) // 8 * 8
+    return (nbatch + 31) // 32 * 32
+
+
+def deferred(nmax: int, nbatch: int) -> bool:
+    return nmax <= DEFER_MAX and batch_capacity(nbatch) > DEFER_MIN_BATCH
+
+
+def jacobi_bytes(n: int, variant: int, defer: bool = False) -> float:
+    """Algorithmic HBM bytes of one sweep: blocked variant, A read and written
+    once per outer step and the eigenvector rows VT once per step, or once per
+    sweep with the deferred update; scalar variant (A in LDS), the two VT rows
+    of every rotation read and written once per step."""
     if variant != 1:
-        return jacobi_bytes_per_sweep(n)
+        return jacobi_bytes_per_sweep(n, defer)
     m = n + (n & 1)
     return (m - 1) * 2.0 * n * n * 8
 
@@ -265,13 +286,15 @@ def jacobi_roofline(records, base) -> dict:
         sweeps = sw.cpu().numpy()
         nsweep += int(sweeps.sum())
         nmat += len(sweeps)
+        dfr = deferred(n, len(sweeps))
         for i in range(len(sweeps)):
             canon += canonical_eigh_flops(int(nao[i]))
             k = int(sweeps[i])
             if k:
                 flops += k * jacobi_flops_per_sweep(int(nao[i]), variant)
-                nbytes += k * jacobi_bytes(int(nao[i]), variant)
-                traffic += JACOBI_TRAFFIC_RATIO[variant] * k * jacobi_bytes(int(nao[i]), variant)
+                nb_i = k * jacobi_bytes(int(nao[i]), variant, dfr)
+                nbytes += nb_i
+                traffic += JACOBI_TRAFFIC_RATIO[variant] * nb_i
     busy_ms = _union_ms(iv)
     total_s = busy_ms * 1e-3
     ncall = len(iv)
@@ -298,11 +321,14 @@ def jacobi_roofline(records, base) -> dict:
     }
 
 
-def jacobi_bytes_per_sweep(n: int) -> float:
+def jacobi_bytes_per_sweep(n: int, defer: bool = False) -> float:
     """HBM bytes one sweep of the blocked Jacobi variant moves for an order-n
     matrix: every outer step reads and writes A and the eigenvector rows VT
-    once (4 n^2 doubles; the 32x32 rotations are L2-resident and small)."""
+    once (4 n^2 doubles; the 32x32 rotations are L2-resident and small), or,
+    with the deferred update, A per step and VT once per sweep."""
     nb = 2 * (-(-n // 32))
+    if defer:
+        return (nb - 1) * 2.0 * n * n * 8 + 2.0 * n * n * 8
     return (nb - 1) * 4.0 * n * n * 8
 
 
@@ -359,7 +385,7 @@ def bench_eigen(args, dev):
     wall = (time.perf_counter() - t0) / args.steps
     assert int(sw.min()) == 1 and int(sw.max()) == 1
     t = float(np.mean(ms)) * 1e-3
-    nbytes = B * jacobi_bytes_per_sweep(n)
+    nbytes = B * jacobi_bytes_per_sweep(n, deferred(n, B))
     flops = B * jacobi_flops_per_sweep(n, 2)
     res = {
         "metric": f"batched Jacobi sweeps/s ({'split' if variant == 3 else 'one-workgroup'} blocked form, FP64)",

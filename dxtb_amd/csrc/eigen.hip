@@ -331,7 +331,9 @@ __device__ __forceinline__ double row_share(double v) {
 // coefficients with row_share moves instead of an LDS round trip.  Each S and
 // Q element has one owner per round (disjoint pairs), so a round needs one
 // wavefront barrier, before the next round reads the new diagonal.
-__device__ bool inner_sweep(WaveLds& W, double thr, bool cross, int lane) {
+template <bool CROSS>
+__device__ bool inner_sweep(WaveLds& W, double thr, int lane) {
+  constexpr bool cross = CROSS;
   bool any = false;
   const int r = lane >> 4;
   const int kk = ((lane & 15) + r) & 15;  // pair of this lane's rotation; its S column pair
@@ -674,7 +676,7 @@ __device__ bool bj_inner_pair(WaveLds& W, const BjMat& M, int k, int s, double t
     W.Q[r * SST + c] = (r == c) ? 1.0 : 0.0;
   }
   wave_sync();
-  const bool any = inner_sweep(W, thr, s > 0, lane);
+  const bool any = inner_sweep<false>(W, thr, lane);  // step 0: all pairs (later steps: bj_inner_cross)
   if (any) {
 #pragma unroll 4
     for (int u = 0; u < 16; ++u) {

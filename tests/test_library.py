@@ -48,16 +48,19 @@ def test_invalid_arguments_rejected_without_gpu():
 def test_jacobi_arguments_without_gpu():
     lib = _lib.load()
     # workspace: a 256-byte argument block, then per matrix of the batch
-    # capacity 32 rows of the order padded to a multiple of 32 (rotations),
-    # 4 doubles of split-sweep state, 48 int rotation flags (block pairs of
-    # the largest order, 1536)
-    assert lib.dxtb_jacobi_max_n() == 1536
-    per = 64 * 32 * 8 + 4 * 8 + 48 * 4
+    # capacity the 32x32 rotations of the order padded to a multiple of 32 for
+    # every step of a sweep (order 33: 2 * 2 - 1 = 3 steps of 64 rows; the
+    # whole sweep's are kept up to order 512), 4 doubles of split-sweep state,
+    # and per step 128 int rotation flags (block pairs of the largest order, 4096)
+    assert lib.dxtb_jacobi_max_n() == 4096
+    per = 3 * 64 * 32 * 8 + 4 * 8 + 3 * 128 * 4
     assert lib.dxtb_jacobi_work_size(2, 33) == 256 + 2 * per
     # capacities: powers of two up to 8, multiples of 8 up to 64, then of 32
     for nb, cap in ((1, 1), (3, 4), (5, 8), (9, 16), (63, 64), (65, 96), (256, 256), (257, 288)):
         assert lib.dxtb_jacobi_work_size(nb, 33) == 256 + cap * per, nb
     assert lib.dxtb_jacobi_work_size(0, 33) == 0
+    # above order 512 the rotations of one step only: order 600, 608 rows
+    assert lib.dxtb_jacobi_work_size(1, 600) == 256 + 608 * 32 * 8 + 4 * 8 + 128 * 4
     assert lib.dxtb_jacobi_max_n() >= 128
     args = (None,) * 5
     assert lib.dxtb_jacobi_eigh(0, 10, *args, 0, 10, 0, 1e-14, None, None, None) == _lib.DXTB_EINVAL
