@@ -203,6 +203,8 @@ def jacobi_flops_per_sweep(n: int, variant: int) -> float:
 # both with the deferred eigenvector update there (256 matrices).  The scalar
 # form (1, A in LDS) is not measured and counted at its algorithmic bytes.
 JACOBI_TRAFFIC_RATIO = {0: 1.103, 1: 1.0, 2: 1.154, 3: 1.103}
+# the same with the deferred eigenvector update, against its algorithmic bytes
+JACOBI_TRAFFIC_RATIO_DEFERRED = {0: 1.170, 1: 1.0, 2: 1.542, 3: 1.170}
 # the library's choice of the deferred eigenvector update (eigen.hip:
 # defer_call): order <= 512 and a batch capacity above 16
 DEFER_MAX, DEFER_MIN_BATCH = 512, 16
@@ -294,7 +296,7 @@ def jacobi_roofline(records, base) -> dict:
                 flops += k * jacobi_flops_per_sweep(int(nao[i]), variant)
                 nb_i = k * jacobi_bytes(int(nao[i]), variant, dfr)
                 nbytes += nb_i
-                traffic += JACOBI_TRAFFIC_RATIO[variant] * nb_i
+                traffic += (JACOBI_TRAFFIC_RATIO_DEFERRED if dfr else JACOBI_TRAFFIC_RATIO)[variant] * nb_i
     busy_ms = _union_ms(iv)
     total_s = busy_ms * 1e-3
     ncall = len(iv)
@@ -325,10 +327,11 @@ def jacobi_bytes_per_sweep(n: int, defer: bool = False) -> float:
     """HBM bytes one sweep of the blocked Jacobi variant moves for an order-n
     matrix: every outer step reads and writes A and the eigenvector rows VT
     once (4 n^2 doubles; the 32x32 rotations are L2-resident and small), or,
-    with the deferred update, A per step and VT once per sweep."""
+    with the deferred update, A per step, VT once per sweep and the sweep's
+    32x32 rotations (n x 32 doubles per step) written and read once."""
     nb = 2 * (-(-n // 32))
-    if defer:
-        return (nb - 1) * 2.0 * n * n * 8 + 2.0 * n * n * 8
+    if defer:  # A per step, VT per sweep, the sweep's rotation log written and read once
+        return (nb - 1) * 2.0 * n * n * 8 + 2.0 * n * n * 8 + 2.0 * (nb - 1) * n * 32 * 8
     return (nb - 1) * 4.0 * n * n * 8
 
 
@@ -399,7 +402,8 @@ def bench_eigen(args, dev):
         "roofline": {"kernel": "k_bj_check + k_bj_inner + k_bj_product + k_jacobi_blocked (one dxtb_jacobi_eigh call)"
                                if variant == 3 else "k_jacobi_blocked", "bound": "hbm", "achieved": nbytes / t / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": nbytes / t / 1e9 / HBM_PEAK_GBS,
-                     "traffic": JACOBI_TRAFFIC_RATIO[variant] * nbytes, "bytes_per_launch": nbytes},
+                     "traffic": (JACOBI_TRAFFIC_RATIO_DEFERRED if deferred(n, B) else JACOBI_TRAFFIC_RATIO)[variant]
+                     * nbytes, "bytes_per_launch": nbytes},
     }
     return res, None
 
@@ -446,7 +450,7 @@ def bench_gfn2_batch(args, dev):
 
     def engine():
         return sv.Engine("gfn2", dev, opts=hscf.SCFOptions(jacobi_variant=args.jacobi_variant),
-                         graphs=args.engine_graphs)
+                         graphs=args.engine_graphs, graph_window_max=args.graph_window_max)
 
     # A step serves the whole batch with a new engine (pool creation is in
     # the timed region).  A warm-up step is the same pass: the device
@@ -513,7 +517,7 @@ def serving_stream(args, dev, nreq: int) -> tuple[dict, dict | None, tuple]:
     solv = workloads.config3_solvation(len(nums))
     n = len(nums)
     eng = sv.Engine("gfn2", dev, opts=hscf.SCFOptions(jacobi_variant=args.jacobi_variant),
-                    graphs=args.engine_graphs)
+                    graphs=args.engine_graphs, graph_window_max=args.graph_window_max)
     # warm-up: a few molecules of every pool class the stream uses
     eng.run(_requests(nums, poss, chg, _stratified(nums, "gfn2", 4), solv), realtime=False,
             progress=_progress("config-3 warm-up"))
@@ -674,11 +678,12 @@ def trace_summary(path: str, steps: int = 1) -> None:
     traced run; no GPU needed."""
     import csv
 
-    rows = []
+    rows, streams = [], []
     with open(path) as fh:
         for d in csv.DictReader(fh):
             rows.append((int(d["Start_Timestamp"]), int(d["End_Timestamp"]),
                          d["Kernel_Name"].replace("(anonymous namespace)::", "")))
+            streams.append(d.get("Stream_Id") or d.get("Queue_Id") or "0")
     if not rows:
         print("no kernels")
         return
@@ -691,11 +696,24 @@ def trace_summary(path: str, steps: int = 1) -> None:
     busy = _union_ms([(a * 1e-6, b * 1e-6) for a, b, _ in rows])
     eig = [(a * 1e-6, b * 1e-6) for a, b, n in rows if any(k in n for k in EIGEN_KERNELS)]
     eig_busy = _union_ms(eig)
+    # whole eigensolver calls, as bench.py times them with events: from a
+    # call's argument-block kernel (k_bj_args) to its finishing kernel
+    # (k_jacobi_blocked) on the same stream, or one k_jacobi_scalar launch
+    calls, open_at = [], {}
+    for (a, b, name), st in sorted(zip(rows, streams)):
+        if "k_bj_args" in name:
+            open_at[st] = a
+        elif "k_jacobi_blocked" in name and st in open_at:
+            calls.append((open_at.pop(st) * 1e-6, b * 1e-6))
+        elif "k_jacobi_scalar" in name:
+            calls.append((a * 1e-6, b * 1e-6))
+    call_busy = _union_ms(calls)
     allsum = sum(t for _, t in tot.values()) * 1e-6
     print(f"kernels {len(rows)}; trace span {span:.1f} ms; device busy (union) {busy:.1f} ms; "
           f"summed kernel time {allsum:.1f} ms; eigensolver busy (union of its kernels) {eig_busy:.1f} ms "
           f"({100 * eig_busy / busy:.1f} % of busy); per step ({steps}): busy {busy / steps:.1f} ms, "
-          f"eigensolver {eig_busy / steps:.1f} ms")
+          f"eigensolver {eig_busy / steps:.1f} ms; eigensolver calls {len(calls)}, busy (union of whole calls, "
+          f"as bench.py's events) {call_busy:.1f} ms")
     print("| kernel | calls | total ms | % of summed | mean us |")
     print("|---|---:|---:|---:|---:|")
     for k, (c, t) in sorted(tot.items(), key=lambda kv: -kv[1][1])[:30]:
@@ -743,6 +761,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--engine-graphs", action="store_true",
                     help="serving engine iterates each pool by replaying a captured hipGraph")
+    ap.add_argument("--graph-window-max", type=int, default=None,
+                    help="with --engine-graphs: replay graphs only for windows of at most this many slots")
     ap.add_argument("--jacobi-variant", type=int, default=3, choices=[0, 2, 3],
                     help="eigensolver form above order 64: 3 split (default), 2 one workgroup per matrix, 0 auto")
     args = ap.parse_args()

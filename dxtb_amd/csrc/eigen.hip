@@ -1000,13 +1000,16 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(5))) vo
                                                    const double* __restrict__ sst, int* __restrict__ rot,
                                                    int s) {
   extern __shared__ double lds_raw[];  // one staging tile per wavefront
-  const int b = blockIdx.y;
+  // grid (matrix, item group): consecutive workgroup ids go to the eight XCDs
+  // in turn, so with a capacity that is a multiple of 8 all workgroups of one
+  // matrix run on one XCD and share its rotations in that XCD's L2
+  const int b = blockIdx.x;
   const JArgsC jc = jconst(ja);
   if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
   const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, rot, jc->defer);
   if (s >= M.nb - 1) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int item = blockIdx.x * PW + wv;
+  const int item = blockIdx.y * PW + wv;
   if (item < bj_nwork(M)) bj_product_item<true>(lds_raw + (size_t)wv * PROD_TILE, M, fstep(M, s), item, s, lane);
 }
 
@@ -1018,7 +1021,9 @@ __global__ __launch_bounds__(AW * 64) void k_bj_apply(int nmax, const JArgs* __r
                                                      double* __restrict__ work, const double* __restrict__ sst,
                                                      int* __restrict__ rot) {
   extern __shared__ double lds_raw[];
-  const int b = blockIdx.y, pc = blockIdx.x;
+  // grid (matrix, panel): the panels of one matrix on one XCD (as
+  // k_bj_product), which reads the sweep's rotation log once into its L2
+  const int b = blockIdx.x, pc = blockIdx.y;
   const JArgsC jc = jconst(ja);
   if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
   const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, rot, jc->defer);
@@ -1084,13 +1089,13 @@ hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax,
         hipLaunchKernelGGL(k_bj_inner, dim3((unsigned)npairs, (unsigned)nbcap), dim3(64), 0, st, nmax, w.ja,
                            w.rot_q, w.sst, w.rot, s);
         if (nwork > 0)  // one block pair with deferred VT: nothing to multiply
-          hipLaunchKernelGGL(k_bj_product, dim3((unsigned)((nwork + PW - 1) / PW), (unsigned)nbcap), dim3(PW * 64),
+          hipLaunchKernelGGL(k_bj_product, dim3((unsigned)nbcap, (unsigned)((nwork + PW - 1) / PW)), dim3(PW * 64),
                              PROD_LDS, st, nmax, w.ja, w.rot_q, w.sst, w.rot, s);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
       }
       if (defer)
-        hipLaunchKernelGGL(k_bj_apply, dim3((unsigned)npairs, (unsigned)nbcap), dim3(AW * 64), apply_lds(nmax), st,
+        hipLaunchKernelGGL(k_bj_apply, dim3((unsigned)nbcap, (unsigned)npairs), dim3(AW * 64), apply_lds(nmax), st,
                            nmax, w.ja, w.rot_q, w.sst, w.rot);
     }
   }
