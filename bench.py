@@ -626,6 +626,7 @@ def cpu_baseline_gfn2(nums, poss, chg, label, solv=None, budget_s=20.0):
     xitorch-style broyden1 at the GPU path's tolerance 1e-8, variational
     autograd forces) on a size-spread sample of the workload for about
     ``budget_s`` seconds."""
+    from dxtb_amd import scf as hscf
     from oracle import scf as oscf
 
     nthreads = os.cpu_count() or 1
@@ -639,7 +640,7 @@ def cpu_baseline_gfn2(nums, poss, chg, label, solv=None, budget_s=20.0):
         sv = {"eps": 78.4, "alpb": False, "kernel": "p16"} if solv is not None and solv[i] else None
         t1 = time.perf_counter()
         oscf.singlepoint(nums[i].tolist(), torch.tensor(poss[i], dtype=torch.float64), "gfn2", charge=float(chg[i]),
-                         tol=1e-8, maxiter=250, forces=True, solvation=sv)
+                         tol=1e-8, maxiter=hscf.SCFOptions().maxiter, forces=True, solvation=sv)
         per.append(time.perf_counter() - t1)
         atoms.append(len(nums[i]))
     dt = time.perf_counter() - t0

@@ -100,11 +100,12 @@ def pool_class(nat: int, nsh: int, nao: int) -> int:
     raise ValueError(f"molecule too large for the serving pools (nao {nao} > {POOL_NAO[-1]})")
 
 
-def default_capacity(nao_max: int, budget_bytes: float = 8e9) -> int:
+def default_capacity(nao_max: int, history: int = 601, budget_bytes: float = 12e9) -> int:
     """Slots of a pool: as many as fill the device (256 workgroups of the
-    per-molecule kernels) within a memory budget per pool."""
+    per-molecule kernels) within a memory budget per pool; ``history`` is
+    the Broyden history length per slot (maxiter + 1)."""
     na, ns, n = class_dims(nao_max)
-    per_slot = 8 * (24 * n * n + 2 * 251 * (ns + 9 * na))  # matrices + Broyden history (upper bound)
+    per_slot = 8 * (24 * n * n + 2 * history * (ns + 9 * na))  # matrices + Broyden history (upper bound)
     return int(max(8, min(256, budget_bytes // per_slot)))
 
 
@@ -376,7 +377,7 @@ class Engine:
 
     def pool(self, c: int) -> SlotPool:
         if c not in self.pools:
-            cap = (self.capacity or {}).get(c, default_capacity(c))
+            cap = (self.capacity or {}).get(c, default_capacity(c, self.opts.maxiter + 1))
             p = SlotPool(self.gfn, c, cap, self.device, self.opts, graphs=self.graphs,
                          graph_window_max=self.graph_window_max)
             torch.cuda.synchronize(p.dev)  # pool arrays were filled on the creating stream
