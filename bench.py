@@ -56,10 +56,13 @@ def integral_flops(la, lb, na, nb):
 
 # measured HBM traffic of the integral forward (all nine class launches) / its
 # algorithmic output bytes: rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1 (scales
-# from a copy of known size), config-1 workload.  Each thread writes short
-# row segments of its block (8-40 bytes per component), so most writes fill
-# only part of a cache line.
-INTEGRAL_FWD_TRAFFIC_RATIO = 2.57
+# from a copy of known size), config-1 workload, by AO layout.  A thread
+# writes a 8-40 byte row segment per component; in the reference layout the
+# columns of one class are interleaved with the other classes', so most lines
+# are written in pieces by several launches; grouped by l, the lanes of a
+# wavefront write adjacent segments and only the lines where two classes'
+# column ranges meet are shared.
+INTEGRAL_FWD_TRAFFIC_RATIO = {"l": 1.450, "reference": 2.607}
 
 
 def bench_integrals(args, dev):
@@ -70,7 +73,7 @@ def bench_integrals(args, dev):
     nmol = args.nmol
     nums, poss, _ = workloads.config_batch(1, count=nmol)
     N, P = workloads.pad(nums, poss)
-    bb = hbasis.build_batch(N, "gfn2", dev)
+    bb = hbasis.build_batch(N, "gfn2", dev, ao_order=args.ao_order)
     pos = torch.as_tensor(P, dtype=torch.float64, device=dev)
     # algorithmic counts over the lower triangle incl. diagonal (unique shell pairs)
     tab = hbasis.shell_table("gfn2")
@@ -129,7 +132,9 @@ def bench_integrals(args, dev):
         "ms_per_step": ms_step,
         "dtype": "f64",
         "config": {"workload": f"config-1 integral microbenchmark: {nmol} GFN2 molecules x 60 atoms",
-                   "shell_pairs": pairs, "nao_max": n},
+                   "shell_pairs": pairs, "nao_max": n,
+                   "ao_order": ("shells grouped by angular momentum (the serving pools' layout)" if args.ao_order == "l"
+                                else "reference (atoms in order; the integral driver's layout)")},
         "roofline": {
             "kernel": "multipole_fwd (9 class launches + output zero fill)",
             "bound": "hbm",
@@ -137,7 +142,7 @@ def bench_integrals(args, dev):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": INTEGRAL_FWD_TRAFFIC_RATIO * out_bytes,
+            "traffic": INTEGRAL_FWD_TRAFFIC_RATIO[args.ao_order] * out_bytes,
             "algorithmic_bytes_per_launch": out_bytes,
             "fwd_ms": fwd,
             "bwd_ms": float(np.mean(bwd_ms)),
@@ -764,6 +769,8 @@ def main():
                     help="serving engine iterates each pool by replaying a captured hipGraph")
     ap.add_argument("--graph-window-max", type=int, default=None,
                     help="with --engine-graphs: replay graphs only for windows of at most this many slots")
+    ap.add_argument("--ao-order", default="l", choices=["l", "reference"],
+                    help="integrals workload: AO layout of the matrices")
     ap.add_argument("--jacobi-variant", type=int, default=3, choices=[0, 2, 3],
                     help="eigensolver form above order 64: 3 split (default), 2 one workgroup per matrix, 0 auto")
     args = ap.parse_args()

@@ -141,6 +141,10 @@ class BatchBasis:
     item_pair: np.ndarray
     class_off: np.ndarray  # (10,)
     dev: dict
+    # AO layout of the matrices: "reference" (atoms in order, each atom's
+    # shells in order) or "l" (the shells grouped by angular momentum: every
+    # s shell, then every p, then every d, each group in reference order)
+    ao_order: str = "reference"
 
     @property
     def nbatch(self) -> int:
@@ -158,13 +162,26 @@ def to_device(a, dev, dtype: torch.dtype | None = None) -> torch.Tensor:
     return t.pin_memory().to(dev, non_blocking=True)
 
 
-def build_batch(numbers, gfn: str, device: torch.device | str, dims: tuple[int, int, int] | None = None
-                ) -> BatchBasis:
+def build_batch(numbers, gfn: str, device: torch.device | str, dims: tuple[int, int, int] | None = None,
+                ao_order: str = "reference") -> BatchBasis:
     """Shell maps and ordered shell-pair lists for a (padded) batch.
 
     ``dims = (nat_max, nsh_max, nao_max)`` pads the batch to fixed sizes (slot
     pools of the serving engine hold molecules of different sizes in arrays
-    of one shape); by default the sizes are the batch maxima."""
+    of one shape); by default the sizes are the batch maxima.
+
+    ``ao_order`` places the shells' basis functions in the matrices: in the
+    reference's order (atoms in order, each atom's shells in order), or with
+    ``"l"`` grouped by angular momentum (every s shell, then every p, then
+    every d, each group in reference order; a shell's 2l+1 functions stay
+    together in the reference's component order).  Shell indices, and every
+    per-shell and per-atom quantity, are the same in both; AO-indexed
+    matrices differ by a symmetric permutation, so energies, charges and
+    forces do not depend on the choice.  With ``"l"`` the pairs of one class
+    (l_A, l_B) fill a contiguous column range of each of their rows, which
+    the integral kernels write and read in whole cache lines."""
+    if ao_order not in ("reference", "l"):
+        raise ValueError(f"ao_order must be 'reference' or 'l', not {ao_order!r}")
     nums = np.atleast_2d(np.asarray(numbers.cpu() if torch.is_tensor(numbers) else numbers, dtype=np.int64))
     if dims is not None:
         if nums.shape[1] > dims[0]:
@@ -205,7 +222,15 @@ def build_batch(numbers, gfn: str, device: torch.device | str, dims: tuple[int, 
         sh_l[mol, pos] = table.l[ush]
     valid = np.arange(nsh_max)[None, :] < nsh[:, None]
     nsph = np.where(valid, 2 * sh_l + 1, 0)
-    sh_ao = (np.cumsum(nsph, 1) - nsph).astype(np.int32)
+    if ao_order == "l":
+        # shells in (l, index) order, padding shells last; offsets in that order
+        perm = np.argsort(np.where(valid, sh_l, 3) * nsh_max + np.arange(nsh_max)[None, :], axis=1, kind="stable")
+        w = np.take_along_axis(nsph, perm, 1)
+        sh_ao = np.zeros_like(w)
+        np.put_along_axis(sh_ao, perm, np.cumsum(w, 1) - w, 1)
+        sh_ao = sh_ao.astype(np.int32)
+    else:
+        sh_ao = (np.cumsum(nsph, 1) - nsph).astype(np.int32)
     nao = nsph.sum(1)
     nao_max = int(max(nao.max(), 1))
     if dims is not None:
@@ -219,7 +244,8 @@ def build_batch(numbers, gfn: str, device: torch.device | str, dims: tuple[int, 
     if tot:
         shell = np.repeat(np.tile(np.arange(nsh_max), B), reps)
         mol = np.repeat(np.repeat(np.arange(B), nsh_max), reps)
-        pos = np.arange(tot) - np.repeat(np.cumsum(nao) - nao, nao)
+        comp = np.arange(tot) - np.repeat(np.cumsum(reps) - reps, reps)  # component within its shell
+        pos = sh_ao[mol, shell] + comp
         ao_shell[mol, pos] = shell
         ao_atom[mol, pos] = sh_atom[mol, shell]
 
@@ -272,5 +298,5 @@ def build_batch(numbers, gfn: str, device: torch.device | str, dims: tuple[int, 
     }
     return BatchBasis(
         gfn, nums, nat, nsh, nao, int(nat_max), nsh_max, nao_max, sh_atom, sh_ush, sh_ao, sh_l,
-        ao_atom, ao_shell, item_mol, item_pair, class_off, d,
+        ao_atom, ao_shell, item_mol, item_pair, class_off, d, ao_order,
     )

@@ -1,10 +1,12 @@
 // Overlap / dipole / quadrupole integrals over contracted Gaussians and their
 // nuclear derivatives (gfx950, FP64).
 //
-// Work decomposition: one thread per (ordered shell pair (A bra, B ket),
-// spherical bra component m_a).  Each thread owns one row segment of the
-// (2l_B+1)-wide block of every output matrix, so the whole matrix (both
-// triangles) is written exactly once and no scatter or mirror pass is needed.
+// Work decomposition of the forward pass: one thread per (ordered shell pair
+// (A bra, B ket), spherical bra component m_a).  Each thread owns one row
+// segment of the (2l_B+1)-wide block of every output matrix, so the whole
+// matrix (both triangles) is written exactly once and no scatter or mirror
+// pass is needed.  The nuclear-derivative pass takes one thread per ordered
+// shell pair and sums over the bra components in registers.
 // The moments are taken about the ket centre B directly, which is the
 // reference's r0 integral followed by its r0->rj shift
 // (integral/types/dipole.py:40-86, quadrupole.py:308-389) computed without the
@@ -136,10 +138,13 @@ __global__ __launch_bounds__(256) void k_multipole_fwd(dxtb_basis_t bs, int off,
   constexpr int NI = LA + 1, NJ = LB + (MP ? 3 : 1);
   constexpr int NCOMB = (LA == 2) ? 3 : 1;
   constexpr int NC = MP ? 10 : 1;
+  // component-major: consecutive lanes take consecutive pairs of the class
+  // (consecutive ket shells of one bra row) with the same bra component, so
+  // with the AO layout grouped by l a wavefront stores adjacent row segments
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int local = tid / NA;
-  if (local >= nitems) return;
-  const int ma = tid - local * NA;
+  if (tid >= nitems * NA) return;
+  const int ma = tid / nitems;
+  const int local = tid - ma * nitems;
   PairCtx c;
   load_pair(bs, pos, off + local, c, ma);
   const double r2 = c.AB[0] * c.AB[0] + c.AB[1] * c.AB[1] + c.AB[2] * c.AB[2];
@@ -274,6 +279,33 @@ struct AdjSrc {
   const double* vqp;
 };
 
+// Sum of v over the run of consecutive lanes of the wavefront that share
+// `key`, returned on the first lane of each run (other lanes: partial sums).
+// Runs are numbered from a ballot of their first lanes, so a key that comes
+// back later in the wavefront starts a new run.  Every lane of the wavefront
+// must call it.
+__device__ __forceinline__ double run_sum_to_head(double v, int key, int lane, bool& head) {
+  const int kp = __shfl_up(key, 1, 64);
+  head = lane == 0 || kp != key;
+  const unsigned long long heads = __ballot(head);
+  const int run = __popcll(heads & ((2ull << lane) - 1));  // runs starting at lanes 0..lane
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double vd = __shfl_down(v, d, 64);
+    const int rd = __shfl_down(run, d, 64);
+    if (lane + d < 64 && rd == run) v += vd;
+  }
+  return v;
+}
+
+// One thread per ordered shell pair (A, B) of the class; the thread loops over
+// the bra components, so each pair's gradient g = d/dR_A (the ket atom gets
+// -g) is summed in registers.  The bra-atom contributions of a wavefront are
+// then summed over its runs of equal (molecule, bra atom) -- the pairs of a
+// class are ordered by molecule, bra shell, ket shell -- and each run issues
+// one atomic per direction; the ket atom takes one atomic per pair.  The
+// (d, d) class, which has few pairs and the most registers, keeps one thread
+// per (pair, bra component), component-major.
 template <int LA, int LB, bool MP, bool SCF>
 __global__ __launch_bounds__(256) void k_multipole_vjp(dxtb_basis_t bs, int off, int nitems,
                                                        const double* __restrict__ pos, AdjSrc src,
@@ -282,16 +314,31 @@ __global__ __launch_bounds__(256) void k_multipole_vjp(dxtb_basis_t bs, int off,
   constexpr int NI = LA + 2, NJ = LB + (MP ? 3 : 1);
   constexpr int NCOMB = (LA == 2) ? 3 : 1;
   constexpr int NC = MP ? 10 : 1;
+  constexpr bool SPLIT = LA == 2 && LB == 2;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int local = tid / NA;
-  if (local >= nitems) return;
-  const int ma = tid - local * NA;
+  const int lane = threadIdx.x & 63;
+  const bool active = tid < (SPLIT ? nitems * NA : nitems);
+  const int ma0 = SPLIT ? tid / nitems : 0;
+  const int local = SPLIT ? tid - ma0 * nitems : tid;
+  const int ma1 = SPLIT ? ma0 + 1 : NA;
+  double g[3] = {0.0, 0.0, 0.0};
+  // run key (molecule, bra atom): sorted over the active lanes, which precede
+  // the inactive ones (the grid's tail), whose keys are unique
+  int key = -1 - lane;
+  bool two = false;  // a pair on one atom adds g and -g to that atom: g = 0
   PairCtx c;
-  load_pair(bs, pos, off + local, c, ma);
+  if (active) {
+    load_pair(bs, pos, off + local, c, 0);
+    key = c.b * bs.nat_max + c.atA;
+    two = c.atA != c.atB;
+  }
+  if (two) {
   const double r2 = c.AB[0] * c.AB[0] + c.AB[1] * c.AB[1] + c.AB[2] * c.AB[2];
-
+  const int row0 = c.row;
   const int n = bs.nao_max;
   const size_t nn = (size_t)n * n;
+  for (int ma = ma0; ma < ma1; ++ma) {
+  c.row = row0 + ma;
   // adjoint of this row segment, transformed to ket Cartesians and raw moments
   double adj[NCB][NC];
 #pragma unroll
@@ -368,7 +415,6 @@ __global__ __launch_bounds__(256) void k_multipole_vjp(dxtb_basis_t bs, int off,
     }
   }
 
-  double g[3] = {0.0, 0.0, 0.0};
   const int npA = bs.ush_np[c.uA], npB = bs.ush_np[c.uB];
   const double* alA = bs.ush_alpha + c.uA * DXTB_MAXP;
   const double* alB = bs.ush_alpha + c.uB * DXTB_MAXP;
@@ -422,14 +468,18 @@ __global__ __launch_bounds__(256) void k_multipole_vjp(dxtb_basis_t bs, int off,
       }
     }
   }
-  if (c.atA != c.atB) {
-    double* gA = grad + ((size_t)c.b * bs.nat_max + c.atA) * 3;
+  }  // bra components
+  }  // pair on two atoms
+  bool head = false;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    const double s = run_sum_to_head(g[x], key, lane, head);
+    if (head && key >= 0) atomic_add_f64(grad + (size_t)key * 3 + x, s);
+  }
+  if (two) {
     double* gB = grad + ((size_t)c.b * bs.nat_max + c.atB) * 3;
 #pragma unroll
-    for (int x = 0; x < 3; ++x) {
-      atomic_add_f64(gA + x, g[x]);
-      atomic_add_f64(gB + x, -g[x]);
-    }
+    for (int x = 0; x < 3; ++x) atomic_add_f64(gB + x, -g[x]);
   }
 }
 
@@ -457,7 +507,7 @@ int launch_vjp(const dxtb_basis_t& bs, const double* pos, const AdjSrc& src, boo
   const int c = 3 * LA + LB;
   const int off = bs.class_off[c], cnt = bs.class_off[c + 1] - off;
   if (cnt <= 0) return 0;
-  const long threads = (long)cnt * (2 * LA + 1);
+  const long threads = (LA == 2 && LB == 2) ? (long)cnt * 5 : cnt;  // per pair; (d, d): per (pair, component)
   const int blk = 256;
   const dim3 grid((unsigned)((threads + blk - 1) / blk));
   if (scf) {

@@ -304,7 +304,8 @@ class GeometryState:
             u = bb.sh_ush[b, : bb.nsh[b]]
             occ = tab_occ[u]
             nel[b] = occ.sum()
-            n0[b, : bb.nao[b]] = np.repeat(occ / (2 * bb.sh_l[b, : bb.nsh[b]] + 1), 2 * bb.sh_l[b, : bb.nsh[b]] + 1)
+            # per AO through the AO -> shell map (any AO order)
+            n0[b, : bb.nao[b]] = (occ / (2 * bb.sh_l[b, : bb.nsh[b]] + 1))[bb.ao_shell[b, : bb.nao[b]]]
         chg = np.zeros(B) if charges is None else np.asarray(
             charges.cpu() if torch.is_tensor(charges) else charges, dtype=np.float64).reshape(B)
         nel = nel - chg
@@ -403,7 +404,7 @@ class GeometryState:
             bb = BatchBasis(
                 ob.gfn, ob.numbers[kc], ob.nat[kc], ob.nsh[kc], ob.nao[kc], ob.nat_max, ob.nsh_max, ob.nao_max,
                 ob.sh_atom[kc], ob.sh_ush[kc], ob.sh_ao[kc], ob.sh_l[kc], ob.ao_atom[kc], ob.ao_shell[kc],
-                ob.item_mol[:0], ob.item_pair[:0], np.zeros(10, dtype=np.int32), dev,
+                ob.item_mol[:0], ob.item_pair[:0], np.zeros(10, dtype=np.int32), dev, ob.ao_order,
             )
         elif (bb.nat_max, bb.nsh_max, bb.nao_max) != (self.bb.nat_max, self.bb.nsh_max, self.bb.nao_max):
             raise ValueError("subset basis must have the padded sizes of the state")
@@ -428,7 +429,8 @@ class GeometryState:
         host = {k: getattr(ob, k)[:hi] for k in self._BB_HOST}
         new.bb = BatchBasis(ob.gfn, host["numbers"], host["nat"], host["nsh"], host["nao"], ob.nat_max, ob.nsh_max,
                             ob.nao_max, host["sh_atom"], host["sh_ush"], host["sh_ao"], host["sh_l"], host["ao_atom"],
-                            host["ao_shell"], ob.item_mol[:0], ob.item_pair[:0], np.zeros(10, dtype=np.int32), dev)
+                            host["ao_shell"], ob.item_mol[:0], ob.item_pair[:0], np.zeros(10, dtype=np.int32), dev,
+                            ob.ao_order)
         new.bs = _lib.basis_struct(new.bb)
         new._jtol = None
         return new

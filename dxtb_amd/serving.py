@@ -39,6 +39,10 @@ POOL_NAO = (48, 96, 144, 192, 256, 320, 384, 448, 512, 640, 768, 1024, 1280, 153
 # padded atom and shell counts of a class, as fractions of its nao; a molecule
 # goes to the smallest class that holds all three of its sizes
 NAT_FRAC, NSH_FRAC = 0.5, 0.75
+# AO layout of the pools' matrices: shells grouped by angular momentum, so the
+# integral kernels of one class (l_A, l_B) write and read contiguous column
+# ranges (the requests' energies and forces do not depend on the layout)
+AO_ORDER = "l"
 
 
 @dataclass
@@ -130,7 +134,7 @@ class SlotPool:
         h2 = np.zeros((capacity, 2), dtype=np.int64) + 1
         pos = np.zeros((capacity, self.dims[0], 3))
         pos[:, 1, 2] = 1.4
-        bb = build_batch(h2, gfn, self.dev, self.dims)
+        bb = build_batch(h2, gfn, self.dev, self.dims, ao_order=AO_ORDER)
         self.state = hscf.GeometryState(bb, torch.as_tensor(pos, device=self.dev), opts=opts)
         self.state.U = torch.eye(nao_max, dtype=torch.float64, device=self.dev).expand(capacity, nao_max,
                                                                                       nao_max).contiguous()
@@ -186,7 +190,7 @@ class SlotPool:
         for i, r in enumerate(reqs):
             N[i, : len(r.numbers)] = r.numbers
             P[i, : len(r.numbers)] = r.positions
-        bb = build_batch(N, self.gfn, self.dev, self.dims)
+        bb = build_batch(N, self.gfn, self.dev, self.dims, ao_order=AO_ORDER)
         chg = np.array([r.charge for r in reqs], dtype=np.float64)
         spins = [r.spin for r in reqs]
         st = hscf.GeometryState(bb, _to_device(P, self.dev), charges=chg, opts=self.opts,
@@ -312,7 +316,7 @@ class SlotPool:
         N = np.zeros((len(rows), na), dtype=np.int64)
         for i, r in enumerate(reqs):
             N[i, : len(r.numbers)] = r.numbers
-        bb = build_batch(N, self.gfn, self.dev, self.dims)
+        bb = build_batch(N, self.gfn, self.dev, self.dims, ao_order=AO_ORDER)
         rows64 = np.asarray(rows, dtype=np.int64)
         sub = self.state.subset(rows64, bb=bb)
         res = hscf.finalize(sub, self.mix.x.index_select(0, _to_device(rows64, self.dev)), iters.copy(),

@@ -82,6 +82,86 @@ def test_vjp_against_oracle_autograd(cuda, geometries, gfn):
     torch.testing.assert_close(grad, ref, atol=1e-10, rtol=0)
 
 
+def test_vjp_config1_batch_against_oracle_autograd(cuda):
+    """The nuclear-derivative pass on a ragged batch: three seeded config-1
+    molecules (60 atoms, GFN2 basis, H/C/N/O/S) and a smaller config-2
+    molecule, random adjoints of S, D and Q (the padding entries too, which
+    the pass must not read), each molecule against oracle autograd.  The
+    per-pair gradients are summed over runs of lanes before the atomics, so
+    this covers long runs of one bra atom, runs cut by wavefront boundaries
+    and by the end of a class.  Tolerance 1e-9 absolute + 1e-10 relative
+    (gradients of O(10) from ~10^5 adjoint-weighted elements)."""
+    nums, poss, _ = workloads.config_batch(1, count=3)
+    n2, p2, _ = workloads.config_batch(2, count=3)
+    mols = list(zip(nums, poss)) + [(n2[2], p2[2])]
+    print(f"case vjp config1 x3 + config2 ({len(n2[2])} atoms)", flush=True)
+    bb, pos = _batch(mols, "gfn2", cuda)
+    B, n = len(mols), bb.nao_max
+    g = torch.Generator().manual_seed(11)
+    aS = torch.randn(B, n, n, generator=g, dtype=torch.float64)
+    aD = torch.randn(B, 3, n, n, generator=g, dtype=torch.float64)
+    aQ = torch.randn(B, 6, n, n, generator=g, dtype=torch.float64)
+    grad = hint.multipole_vjp(bb, pos, aS.to(cuda), aD.to(cuda), aQ.to(cuda)).cpu()
+    for i, (z, p0) in enumerate(mols):
+        b = obasis.build([int(x) for x in z], "gfn2")
+        k = b.nao
+        p = torch.tensor(p0, dtype=torch.float64, requires_grad=True)
+        s, d, q = oint.multipole_matrices(b, p)
+        obj = (aS[i, :k, :k] * s).sum() + (aD[i, :, :k, :k] * d).sum() + (aQ[i, :, :k, :k] * q).sum()
+        (ref,) = torch.autograd.grad(obj, p)
+        torch.testing.assert_close(grad[i, : len(z)], ref, atol=1e-9, rtol=1e-10)
+        assert grad[i, len(z):].abs().max().item() == 0.0 if len(z) < grad.shape[1] else True
+
+
+def _ao_perm(ref, bl, b):
+    """AO index in layout ``bl`` of every AO of the reference layout."""
+    na = int(ref.nao[b])
+    perm = np.empty(na, dtype=np.int64)
+    for s in range(int(ref.nsh[b])):
+        k = 2 * int(ref.sh_l[b, s]) + 1
+        perm[ref.sh_ao[b, s] : ref.sh_ao[b, s] + k] = np.arange(bl.sh_ao[b, s], bl.sh_ao[b, s] + k)
+    return torch.as_tensor(perm)
+
+
+@pytest.mark.parametrize("gfn", ["gfn1", "gfn2"])
+def test_l_layout_equals_reference_layout(cuda, gfn):
+    """The integrals in the AO layout grouped by angular momentum (the
+    serving pools' layout) are the reference layout's, permuted: every element
+    is computed by the same arithmetic, so the match is exact; the nuclear
+    derivatives agree up to the order of the atomic additions."""
+    nums, poss, _ = workloads.config_batch(1, count=3)
+    n2, p2, _ = workloads.config_batch(2, count=2)
+    mols = list(zip(nums, poss)) + [(n2[1], p2[1])]
+    print(f"case l layout {gfn}", flush=True)
+    N, P = workloads.pad([m[0] for m in mols], [m[1] for m in mols])
+    pos = torch.as_tensor(P, dtype=torch.float64, device=cuda)
+    ref = hbasis.build_batch(N, gfn, cuda)
+    bl = hbasis.build_batch(N, gfn, cuda, ao_order="l")
+    out_r = hint.multipole_forward(bb=ref, positions=pos)[:3]
+    out_l = hint.multipole_forward(bb=bl, positions=pos)[:3]
+    n = ref.nao_max
+    g = torch.Generator().manual_seed(5)
+    adj_r = [torch.randn(len(mols), *s, n, n, generator=g, dtype=torch.float64) for s in ((), (3,), (6,))]
+    adj_l = [torch.zeros_like(a) for a in adj_r]
+    for b in range(len(mols)):
+        pm = _ao_perm(ref, bl, b)
+        na = len(pm)
+        ix = torch.arange(na)
+        for r, l in zip(out_r, out_l):
+            rr, ll = r[b].cpu(), l[b].cpu()
+            if rr.dim() == 2:
+                rr, ll = rr[None], ll[None]
+            assert torch.equal(ll[:, pm[:, None], pm[None, :]], rr[:, :na, :na])
+        for a_r, a_l in zip(adj_r, adj_l):
+            ar, al = a_r[b], a_l[b]
+            if ar.dim() == 2:
+                ar, al = ar[None], al[None]
+            al[:, pm[:, None], pm[None, :]] = ar[:, ix[:, None], ix[None, :]]
+    g_r = hint.multipole_vjp(ref, pos, *[a.to(cuda) for a in adj_r]).cpu()
+    g_l = hint.multipole_vjp(bl, pos, *[a.to(cuda) for a in adj_l]).cpu()
+    torch.testing.assert_close(g_l, g_r, atol=1e-11, rtol=1e-12)
+
+
 def test_autograd_function(cuda, geometries):
     m = geometries["H2O"]
     bb, pos = _batch([(np.array(m["numbers"]), np.array(m["positions"]))], "gfn2", cuda)
