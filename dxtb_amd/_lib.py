@@ -84,6 +84,7 @@ class DxtbGbParams(ctypes.Structure):
 _SIGNATURES: dict[str, list] = {
     "dxtb_multipole_fwd": [POINTER(DxtbBasis)] + [c_void_p] * 6 + [c_int, c_void_p],
     "dxtb_multipole_vjp": [POINTER(DxtbBasis), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "dxtb_pair_bounds": [POINTER(DxtbBasis), c_void_p, c_void_p, c_void_p, c_void_p],
     "dxtb_setup": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 9 + [c_void_p],
     "dxtb_fock": [POINTER(DxtbBasis)] + [c_void_p] * 12 + [c_double, c_void_p, c_void_p],
     "dxtb_charges": [POINTER(DxtbBasis)] + [c_void_p] * 17,

@@ -312,15 +312,23 @@ __device__ __forceinline__ int acc_row(int lane, int i) { return (lane >> 4) + 4
 // pair, identical results), reads the other rotations it needs from LDS, then
 // loads all of its S and Q elements before it writes any of them (the 2x2
 // tiles and the Q row pairs of one step are disjoint).
-// Value of lane L of this lane's row of 16 lanes (DPP row_share, two 32-bit
-// moves per double).  Every lane of the wavefront must be active.
-template <int L>
-__device__ __forceinline__ double row_share(double v) {
+// A double moved between the lanes of each row of 16 by the DPP control CTRL
+// (two 32-bit moves).  Every lane of the wavefront must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x150 + L, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x150 + L, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
   return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
 }
+
+// Value of lane L of this lane's row of 16 lanes (DPP row_share).
+template <int L>
+__device__ __forceinline__ double row_share(double v) { return dpp_f64<0x150 + L>(v); }
+
+// Value of lane x + 1 (mod 16) of this lane's row of 16 lanes (DPP
+// row_ror:15; checked on the MI355X: lane x of every row receives lane x + 1).
+__device__ __forceinline__ double row_next(double v) { return dpp_f64<0x12F>(v); }
 
 // One sweep of the 32x32 subproblem in W by one wavefront.  Lane j of row r
 // (lane = 16 r + j) computes the rotation of pair kk = (j + r) mod 16 and owns
@@ -574,7 +582,6 @@ __device__ bool bj_inner_cross(WaveLds& W, const BjMat& M, int k, int s, double 
   // this lane holds the diagonal tile (x, x) when x is one of its row slots
   const int ud = x - 4 * g;
   const bool diag = ud >= 0 && ud < 4;
-  const int nxt = (lane & 48) | ((lane + 1) & 15);  // slot x + 1, same row group
   const int down = (((g + 1) & 3) << 4) | x;        // row slot 4 g + 4, same column
   const int down_nxt = (((g + 1) & 3) << 4) | ((x + 1) & 15);
   double* cs = W.S;  // (c, s) of the 16 pairs
@@ -615,17 +622,19 @@ __device__ bool bj_inner_cross(WaveLds& W, const BjMat& M, int k, int s, double 
       }
       wave_sync();  // the coefficients are read before the next round writes them
     }
-    // advance sigma: J-side columns take slot x + 1, J-side rows slot j + 1
+    // advance sigma: J-side columns take slot x + 1 (a rotation within the
+    // row of 16 lanes, DPP), J-side rows slot j + 1 (the last one from the
+    // next row group, the only moves across rows)
 #pragma unroll
-    for (int u = 0; u < 4; ++u) bij[u] = __shfl(bij[u], nxt, 64);
+    for (int u = 0; u < 4; ++u) bij[u] = row_next(bij[u]);
 #pragma unroll
-    for (int v = 0; v < 8; ++v) qj[v] = __shfl(qj[v], nxt, 64);
+    for (int v = 0; v < 8; ++v) qj[v] = row_next(qj[v]);
     const double tji = __shfl(bji[0], down, 64);
     const double tjj = __shfl(bjj[0], down_nxt, 64);
 #pragma unroll
     for (int u = 0; u < 3; ++u) {
       bji[u] = bji[u + 1];
-      bjj[u] = __shfl(bjj[u + 1], nxt, 64);
+      bjj[u] = row_next(bjj[u + 1]);
     }
     bji[3] = tji;
     bjj[3] = tjj;

@@ -543,6 +543,96 @@ int vjp_all(const dxtb_basis_t& b, const double* pos, const AdjSrc& src, bool mp
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// shell-pair screening bounds
+// ---------------------------------------------------------------------------
+// Upper bound of the largest |element| of the spherical block of a shell pair
+// for the multipole orders e = 0, 1, 2 about the origin C (the reference's
+// _bounds, impls/pairs.py:411-494): per primitive pair
+//   |c_i c_j| exp(-mu R^2) sum_{i<=la, j<=lb, k<=e} C(la,i) C(lb,j) C(e,k)
+//     |PA|^i |PB|^j |PC|^k 2 pi Gamma((n+3)/2) p^(-(n+3)/2),
+// n = (la-i) + (lb-j) + (e-k), |PA| = b R / p, |PB| = a R / p and |PC| <=
+// max(|AC|, |BC|), summed over the primitive pairs and scaled by the largest
+// absolute row sums of the two spherical transforms (1, 1, 2 for s, p, d)
+// times 1 + 1e-10.  Both orders of a pair are evaluated as the reference's
+// lower triangle orders them (bra = the larger shell index), so (A, B) and
+// (B, A) get bit-identical bounds.  One thread per work-list item; out[4 i +
+// 3] is 1 when both shells sit at the same position.
+__device__ __forceinline__ double binom2(int n, int k) { return (n == 2 && k == 1) ? 2.0 : 1.0; }
+__device__ __forceinline__ double sph_row_norm(int l) { return l == 2 ? 2.0 : 1.0; }
+
+__global__ __launch_bounds__(256) void k_pair_bounds(dxtb_basis_t bs, int nitems, const double* __restrict__ pos,
+                                                     double cx, double cy, double cz, double* __restrict__ out) {
+  const int item = blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= nitems) return;
+  const int b = bs.item_mol[item];
+  const int pr = bs.item_pair[item];
+  const int i0 = pr >> 16, j0 = pr & 0xffff;
+  const int sa = max(i0, j0), sb = min(i0, j0);
+  const int base = b * bs.nsh_max;
+  const int atA = bs.sh_atom[base + sa], atB = bs.sh_atom[base + sb];
+  const int uA = bs.sh_ush[base + sa], uB = bs.sh_ush[base + sb];
+  const int la = bs.ush_l[uA], lb = bs.ush_l[uB];
+  const double* pA = pos + ((size_t)b * bs.nat_max + atA) * 3;
+  const double* pB = pos + ((size_t)b * bs.nat_max + atB) * 3;
+  const double c[3] = {cx, cy, cz};
+  double r2 = 0.0, ac2 = 0.0, bc2 = 0.0;
+  bool same = true;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    const double d = pB[x] - pA[x], da = pA[x] - c[x], db = pB[x] - c[x];
+    r2 += d * d;
+    ac2 += da * da;
+    bc2 += db * db;
+    same = same && pA[x] == pB[x];
+  }
+  const double r = sqrt(r2), xc = fmax(sqrt(ac2), sqrt(bc2));
+  const double xcp[3] = {1.0, xc, xc * xc};
+  // 2 pi Gamma((n + 3) / 2), n = 0..6
+  constexpr double SQPI = 1.7724538509055160273;
+  constexpr double TPI = 6.283185307179586477;
+  const double g[7] = {TPI * 0.5 * SQPI, TPI * 1.0, TPI * 0.75 * SQPI, TPI * 2.0,
+                       TPI * 1.875 * SQPI, TPI * 6.0, TPI * 6.5625 * SQPI};
+  const int npA = bs.ush_np[uA], npB = bs.ush_np[uB];
+  const double* alA = bs.ush_alpha + uA * DXTB_MAXP;
+  const double* alB = bs.ush_alpha + uB * DXTB_MAXP;
+  const double* coA = bs.ush_coeff + uA * DXTB_MAXP;
+  const double* coB = bs.ush_coeff + uB * DXTB_MAXP;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < npA; ++i) {
+    const double a = alA[i];
+    for (int j = 0; j < npB; ++j) {
+      const double bb = alB[j];
+      const double p = a + bb, oop = 1.0 / p, sq = sqrt(oop);
+      const double pref = fabs(coA[i] * coB[j]) * exp(-a * bb * oop * r * r);
+      const double xa = bb * oop * r, xb = a * oop * r;
+      double mom[7];
+      double pw = oop * sq;  // p^(-3/2)
+#pragma unroll
+      for (int n = 0; n < 7; ++n) {
+        mom[n] = g[n] * pw;
+        pw *= sq;
+      }
+      const double xap[3] = {1.0, xa, xa * xa}, xbp[3] = {1.0, xb, xb * xb};
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        double poly = 0.0;
+        for (int ii = 0; ii <= la; ++ii)
+          for (int jj = 0; jj <= lb; ++jj)
+#pragma unroll
+            for (int k = 0; k <= e; ++k)
+              poly += binom2(la, ii) * binom2(lb, jj) * binom2(e, k) * xap[ii] * xbp[jj] * xcp[k] *
+                      mom[(la - ii) + (lb - jj) + (e - k)];
+        acc[e] += pref * poly;
+      }
+    }
+  }
+  const double sph = sph_row_norm(la) * sph_row_norm(lb) * (1.0 + 1e-10);
+#pragma unroll
+  for (int e = 0; e < 3; ++e) out[(size_t)item * 4 + e] = acc[e] * sph;
+  out[(size_t)item * 4 + 3] = same ? 1.0 : 0.0;
+}
+
 bool basis_ok(const dxtb_basis_t* b) {
   if (b == nullptr) return false;
   if (b->nbatch <= 0 || b->nao_max <= 0 || b->nsh_max <= 0 || b->nat_max <= 0) return false;
@@ -588,4 +678,15 @@ extern "C" int dxtb_forces_multipole(const dxtb_basis_t* bas, const double* pos,
   if ((vdp == nullptr) != (vqp == nullptr)) return DXTB_EINVAL;
   AdjSrc src{nullptr, nullptr, nullptr, P, W, fsh, vsh, vdp, vqp};
   return vjp_all(*bas, pos, src, vdp != nullptr, true, grad, stream);
+}
+
+extern "C" int dxtb_pair_bounds(const dxtb_basis_t* bas, const double* pos, const double* origin, double* out,
+                                hipStream_t stream) {
+  if (!basis_ok(bas) || pos == nullptr || out == nullptr || bas->ush_l == nullptr) return DXTB_EINVAL;
+  const int n = bas->class_off[9];
+  if (n <= 0) return 0;
+  const double cx = origin ? origin[0] : 0.0, cy = origin ? origin[1] : 0.0, cz = origin ? origin[2] : 0.0;
+  hipLaunchKernelGGL(k_pair_bounds, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *bas, n, pos, cx, cy,
+                     cz, out);
+  return (int)hipGetLastError();
 }

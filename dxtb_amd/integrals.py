@@ -27,11 +27,33 @@ def _check_positions(bb: BatchBasis, positions: torch.Tensor) -> torch.Tensor:
     return pos.detach().contiguous()
 
 
+def work_basis(bb: BatchBasis, positions: torch.Tensor, screening_threshold: float | None = None,
+               pairs: torch.Tensor | None = None) -> BatchBasis:
+    """The basis whose work lists the kernels run: ``bb`` itself, or with
+    shell-pair screening (``screening.py``) only the pairs of the selection
+    ``pairs`` or of ``screening_threshold`` on these positions (detached:
+    dropped blocks stay zero and get no gradient), as the reference's
+    assemble_matrix takes them (impls/pairs.py:228-340)."""
+    if pairs is not None and screening_threshold is not None:
+        raise ValueError("Pass either `screening_threshold` or `pairs`, not both.")
+    if pairs is None and screening_threshold is None:
+        return bb
+    from .screening import screened_basis, select_pairs
+
+    if pairs is None:
+        pairs = select_pairs(bb, _check_positions(bb, positions), screening_threshold)
+    return screened_basis(bb, pairs)
+
+
 def multipole_forward(bb: BatchBasis, positions: torch.Tensor, multipoles: bool = True,
-                      fsh: torch.Tensor | None = None, traceless: bool = True):
+                      fsh: torch.Tensor | None = None, traceless: bool = True,
+                      screening_threshold: float | None = None, pairs: torch.Tensor | None = None):
     """S, D, Q (None if ``multipoles`` is False) and H0 (if ``fsh`` is given).
-    ``traceless=False`` returns the raw ket-centred second moments."""
+    ``traceless=False`` returns the raw ket-centred second moments.
+    ``screening_threshold`` or ``pairs`` skip the screened shell pairs
+    (``work_basis``)."""
     lib = _lib.load()
+    bb = work_basis(bb, positions, screening_threshold, pairs)
     pos = _check_positions(bb, positions)
     dev = pos.device
     n = bb.nao_max
@@ -49,9 +71,12 @@ def multipole_forward(bb: BatchBasis, positions: torch.Tensor, multipoles: bool 
     return S, D, Q, H0
 
 
-def multipole_vjp(bb: BatchBasis, positions: torch.Tensor, adjS, adjD=None, adjQ=None) -> torch.Tensor:
-    """d/dR [sum(adjS*S) + sum(adjD*D) + sum(adjQ*Q)], shape (B, nat_max, 3)."""
+def multipole_vjp(bb: BatchBasis, positions: torch.Tensor, adjS, adjD=None, adjQ=None,
+                  screening_threshold: float | None = None, pairs: torch.Tensor | None = None) -> torch.Tensor:
+    """d/dR [sum(adjS*S) + sum(adjD*D) + sum(adjQ*Q)], shape (B, nat_max, 3)
+    (of the screened matrices with ``screening_threshold`` or ``pairs``)."""
     lib = _lib.load()
+    bb = work_basis(bb, positions, screening_threshold, pairs)
     pos = _check_positions(bb, positions)
     grad = torch.zeros_like(pos)
     st = _lib.basis_struct(bb)
@@ -91,5 +116,9 @@ class MultipoleFunction(torch.autograd.Function):
         return g.reshape(ctx.shape), None, None
 
 
-def multipoles(bb: BatchBasis, positions: torch.Tensor, multipoles: bool = True):
-    return MultipoleFunction.apply(positions, bb, multipoles)
+def multipoles(bb: BatchBasis, positions: torch.Tensor, multipoles: bool = True,
+               screening_threshold: float | None = None, pairs: torch.Tensor | None = None):
+    """Differentiable (S, D, Q); with ``screening_threshold`` or ``pairs`` the
+    screened matrices (the selection is made once, from the detached
+    positions)."""
+    return MultipoleFunction.apply(positions, work_basis(bb, positions, screening_threshold, pairs), multipoles)

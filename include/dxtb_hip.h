@@ -69,6 +69,19 @@ int dxtb_multipole_fwd(const dxtb_basis_t* bas, const double* pos, const double*
 int dxtb_multipole_vjp(const dxtb_basis_t* bas, const double* pos, const double* adjS,
                        const double* adjD, const double* adjQ, double* grad, hipStream_t stream);
 
+/* Shell-pair screening bounds of the work lists (screening is off unless a
+ * caller asks for it, as in the reference).  For every item of bas (class
+ * order, class_off[9] items) out (nitems, 4) receives the rigorous upper
+ * bounds of the largest |element| of the pair's spherical block for the
+ * multipole orders 0, 1 and 2 about origin (a host pointer to 3 doubles, NULL:
+ * the Cartesian origin), then 1.0 if both shells sit at the same position
+ * (else 0.0).  Replaces the reference's _bounds and the same-position test of
+ * _screen (src/dxtb/_src/integral/driver/pytorch/impls/pairs.py:411-541);
+ * the caller keeps the items with any bound above its threshold or at one
+ * position (select_pairs, pairs.py:544-614). */
+int dxtb_pair_bounds(const dxtb_basis_t* bas, const double* pos, const double* origin, double* out,
+                     hipStream_t stream);
+
 /* Method parameters as device tables.  Per-unique-shell arrays use the row
  * order of dxtb_basis_t.ush_*; per-element arrays are indexed by the atomic
  * number (0..86, row 0 unused). */

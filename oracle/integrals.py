@@ -10,6 +10,11 @@ Restates the reference PyTorch integral driver:
 * lower-triangle shell pairs, halved diagonal blocks, mirror --
   ``impls/pairs.py:92-149, 207-225, 322-399``
 * unit self-overlap rescaling of the coefficients -- ``impls/pairs.py:181-204``
+* shell-pair screening: the rigorous block bound of ``impls/pairs.py:411-494``
+  (``_bounds``) and the selection of ``impls/pairs.py:497-541`` (``_screen``:
+  pairs at one position always kept, else kept when the bound of any order
+  0, 1, 2 exceeds the threshold), applied to the lower-triangle pairs
+  before the mirror (``impls/pairs.py:322-399``)
 * container post-processing: snorm normalisation (``integral/base.py:152-174``,
   ``integral/utils.py``), r0->rj shift of the dipole (``integral/types/dipole.py:40-86``),
   9->6 reduction, r0r0->rjrj shift and traceless quadrupole
@@ -142,14 +147,92 @@ def _normalized_coeff(l, a, c):
     return c / (torch.sqrt(torch.clamp(s, min=eps)) + eps)
 
 
-def raw_matrices(basis: Basis, positions: torch.Tensor) -> torch.Tensor:
-    """Raw (origin 0) S, D(3), Q(6 unique) matrices, shape (10, nao, nao):
-    lower-triangle shell pairs, halved diagonal blocks, mirror (pairs.py)."""
-    nao = basis.nao
+def sph_norm(l: int) -> float:
+    """Largest absolute row sum of the Cartesian-to-spherical transform of l
+    (``_SPH_NORM``, impls/pairs.py:407-408)."""
+    return float(TRAFO[l].abs().sum(-1).max())
+
+
+def _normalized_all(basis: Basis) -> dict:
     coeffs = {}
     for sh in basis.shells:
         if sh.ush not in coeffs:
             coeffs[sh.ush] = _normalized_coeff(sh.l, sh.alpha, sh.coeff)
+    return coeffs
+
+
+def pair_bound(la, lb, alpha_a, coeff_a, alpha_b, coeff_b, ra, rb, orders=(0, 1, 2), origin=None):
+    """Upper bounds of the largest |element| of the spherical block of one
+    shell pair (bra a at ra, ket b at rb) for each multipole order e in
+    ``orders`` about ``origin`` (default 0), following impls/pairs.py:411-494:
+    per primitive pair |c_i c_j| exp(-mu R^2) times
+    sum_{i<=la, j<=lb, k<=e} C(la,i) C(lb,j) C(e,k) |PA|^i |PB|^j |PC|^k
+    2 pi Gamma((n+3)/2) p^(-(n+3)/2), n = (la-i)+(lb-j)+(e-k), with |PA| =
+    b R / p, |PB| = a R / p, |PC| <= max(|AC|, |BC|); summed over the
+    primitive pairs and scaled by the two transforms' largest row sums
+    (times 1 + 1e-10)."""
+    c = torch.zeros(3, dtype=ra.dtype) if origin is None else torch.as_tensor(origin, dtype=ra.dtype).reshape(3)
+    a, b = alpha_a[:, None], alpha_b[None, :]
+    p = a + b
+    cc = (coeff_a[:, None] * coeff_b[None, :]).abs()
+    mu = a * b / p
+    sph = sph_norm(la) * sph_norm(lb) * (1.0 + 1e-10)
+    r = (rb - ra).norm()
+    xc = torch.maximum((ra - c).norm(), (rb - c).norm())
+    xa, xb = b / p * r, a / p * r
+    pref = cc * torch.exp(-mu * r * r)
+    emax = max(orders)
+    mom = [2.0 * math.pi * math.gamma((n + 3) / 2) * p ** (-(n + 3) / 2) for n in range(la + lb + emax + 1)]
+    out = []
+    for e in orders:
+        poly = torch.zeros_like(p)
+        for i in range(la + 1):
+            for j in range(lb + 1):
+                for k in range(e + 1):
+                    n = (la - i) + (lb - j) + (e - k)
+                    poly = poly + (math.comb(la, i) * math.comb(lb, j) * math.comb(e, k)
+                                   * xa**i * xb**j * xc**k * mom[n])
+        out.append((pref * poly).sum() * sph)
+    return torch.stack(out)
+
+
+def pair_bounds(basis: Basis, positions: torch.Tensor, orders=(0, 1, 2), origin=None) -> torch.Tensor:
+    """``pair_bound`` of every shell pair, (len(orders), nsh, nsh), computed
+    for the lower triangle (bra = the larger shell index, as the reference's
+    pair plan orders them) and mirrored."""
+    coeffs = _normalized_all(basis)
+    nsh = basis.nsh
+    out = torch.zeros(len(orders), nsh, nsh, dtype=positions.dtype)
+    for i, shi in enumerate(basis.shells):
+        for j in range(i + 1):
+            shj = basis.shells[j]
+            bd = pair_bound(shi.l, shj.l, shi.alpha, coeffs[shi.ush], shj.alpha, coeffs[shj.ush],
+                            positions[shi.atom], positions[shj.atom], orders, origin)
+            out[:, i, j] = bd
+            out[:, j, i] = bd
+    return out
+
+
+def screen_keep(basis: Basis, positions: torch.Tensor, threshold: float, origin=None) -> torch.Tensor:
+    """(nsh, nsh) bool: the pairs ``_screen`` keeps (impls/pairs.py:497-541):
+    both shells at one position, or the bound of any order 0, 1, 2 above
+    ``threshold``."""
+    pos = positions.detach()
+    bd = pair_bounds(basis, pos, (0, 1, 2), origin)
+    at = basis.shell_atom()
+    same = (pos[at][:, None, :] == pos[at][None, :, :]).all(-1)
+    return same | (bd > threshold).any(0)
+
+
+def raw_matrices(basis: Basis, positions: torch.Tensor, screening_threshold: float | None = None,
+                 origin=None) -> torch.Tensor:
+    """Raw (origin 0) S, D(3), Q(6 unique) matrices, shape (10, nao, nao):
+    lower-triangle shell pairs, halved diagonal blocks, mirror (pairs.py).
+    With ``screening_threshold`` the pairs that ``screen_keep`` drops
+    (bounds about ``origin``) are zero."""
+    nao = basis.nao
+    coeffs = _normalized_all(basis)
+    keep = None if screening_threshold is None else screen_keep(basis, positions, screening_threshold, origin)
     # group lower-triangle pairs by ordered unique-shell class
     classes: dict = {}
     for i, shi in enumerate(basis.shells):
@@ -167,6 +250,8 @@ def raw_matrices(basis: Basis, positions: torch.Tensor) -> torch.Tensor:
         rb = positions[torch.tensor([basis.shells[k].atom for k in jj.tolist()])]
         blk = pair_blocks(sa.l, sb.l, sa.alpha, coeffs[ua], sb.alpha, coeffs[ub], ra, rb)
         w = torch.where(ii == jj, 0.5, 1.0).to(positions.dtype)
+        if keep is not None:
+            w = w * keep[ii, jj].to(positions.dtype)
         blk = blk * w.view(-1, 1, 1, 1)
         aoa = torch.tensor([basis.shells[k].ao for k in ii.tolist()])
         aob = torch.tensor([basis.shells[k].ao for k in jj.tolist()])
@@ -180,11 +265,12 @@ def raw_matrices(basis: Basis, positions: torch.Tensor) -> torch.Tensor:
     return tri + tri.transpose(-1, -2)
 
 
-def multipole_matrices(basis: Basis, positions: torch.Tensor):
+def multipole_matrices(basis: Basis, positions: torch.Tensor, screening_threshold: float | None = None):
     """Final S (nao, nao), D (3, nao, nao) centred on the ket atom and
     traceless Q (6, nao, nao; xx, yx, yy, zx, zy, zz) centred on the ket atom,
-    following integral/container.py:182-535."""
-    raw = raw_matrices(basis, positions)
+    following integral/container.py:182-535 (screened raw integrals with
+    ``screening_threshold``)."""
+    raw = raw_matrices(basis, positions, screening_threshold)
     s0 = raw[0]
     d = s0.diagonal()
     norm = torch.where(d == 0.0, torch.zeros_like(d), d.pow(-0.5))
