@@ -87,6 +87,7 @@ _SIGNATURES: dict[str, list] = {
     "dxtb_pair_bounds": [POINTER(DxtbBasis), c_void_p, c_void_p, c_void_p, c_void_p],
     "dxtb_setup": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 9 + [c_void_p],
     "dxtb_fock": [POINTER(DxtbBasis)] + [c_void_p] * 12 + [c_double, c_void_p, c_void_p],
+    "dxtb_charges_work_size": [c_int, c_int],
     "dxtb_charges": [POINTER(DxtbBasis)] + [c_void_p] * 17,
     "dxtb_potential": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 14 + [c_void_p],
     "dxtb_fermi": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p],
@@ -101,8 +102,8 @@ _SIGNATURES: dict[str, list] = {
     "dxtb_jacobi_eigh_noalloc": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                                  c_double, c_void_p, c_void_p, c_void_p],
     "dxtb_forces_classical": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 19,
-    "dxtb_broyden_work_size": [c_int, c_int],
-    "dxtb_broyden_step": [c_int, c_int, c_int] + [c_void_p] * 11 + [c_double] * 4 + [c_int, c_void_p],
+    "dxtb_broyden_work_size": [c_int, c_int, c_int],
+    "dxtb_broyden_step": [c_int, c_int, c_int] + [c_void_p] * 11 + [c_double] * 4 + [c_int, c_int, c_void_p],
     "dxtb_gemm_batched": [c_int, c_int, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_double,
                           c_void_p, c_void_p, c_int, c_int, c_double, c_void_p],
 }
@@ -132,7 +133,8 @@ def load() -> ctypes.CDLL:
     return _lib
 
 
-_RESTYPES = {"dxtb_jacobi_work_size": c_size_t, "dxtb_broyden_work_size": c_size_t}
+_RESTYPES = {"dxtb_jacobi_work_size": c_size_t, "dxtb_broyden_work_size": c_size_t,
+             "dxtb_charges_work_size": c_size_t}
 
 
 def check(rc: int, what: str) -> None:

@@ -1,7 +1,8 @@
 // Per-geometry set-up (coordination numbers, H0 shell-pair factors, repulsion,
 // AES2 radii) and the per-iteration electrostatic potentials (ES2, ES3, AES2).
-// All O(nat^2)/O(nsh^2) streaming work in FP64; one thread per atom or shell
-// of a molecule, the partner loop runs over the molecule's own atoms/shells.
+// All O(nat^2)/O(nsh^2) streaming work in FP64; the set-up kernels take one
+// thread per atom or shell pair, the potentials a group of lanes per shell or
+// atom.  Partner loops run over the molecule's own atoms/shells.
 
 #include "common.h"
 #include "pairfuncs.h"
@@ -141,36 +142,58 @@ __global__ void k_mrad(int B, int nat_max, dxtb_params_t par, const int* __restr
   mrad[t] = rad + (par.aes_rmax - rad) / (1.0 + t1);
 }
 
+// ---- per-iteration potentials: PG lanes per shell or atom -----------------
+// Each shell (atom) of a window is served by a group of PG lanes that split
+// its partner loop; the group's partial sums are combined by shuffles.  A
+// window of a few large molecules then still spreads over the device, and a
+// lone molecule's partner loops run PG times shorter.
+constexpr int PG = 16;        // lanes per shell / atom (a power of two within a wavefront)
+constexpr int POT_BLK = 256;  // threads per workgroup: POT_BLK / PG targets
+
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = PG / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 // ---- shell potential: ES2 gamma*q (+ GFN2 shell ES3) ----------------------
-__global__ void k_pot_shell(dxtb_basis_t bs, dxtb_params_t par, const int* __restrict__ numbers,
-                            const double* __restrict__ pos, const int* __restrict__ nsh,
-                            const double* __restrict__ qsh, double* __restrict__ vsh,
-                            double* __restrict__ esh) {
+__global__ __launch_bounds__(POT_BLK) void k_pot_shell(dxtb_basis_t bs, dxtb_params_t par,
+                                                       const double* __restrict__ pos, const int* __restrict__ nsh,
+                                                       const double* __restrict__ qsh, double* __restrict__ vsh,
+                                                       double* __restrict__ esh) {
   const int ns = bs.nsh_max;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= bs.nbatch * ns) return;
-  const int b = t / ns, s = t - b * ns;
-  if (s >= nsh[b]) {
+  const long t = ((long)blockIdx.x * POT_BLK + threadIdx.x) / PG;  // target shell of this lane's group
+  const int g = threadIdx.x & (PG - 1);
+  const bool live = t < (long)bs.nbatch * ns;
+  const int b = live ? (int)(t / ns) : 0, s = live ? (int)(t - (long)b * ns) : 0;
+  const int n = live ? nsh[b] : 0;
+  const bool act = live && s < n;
+  const int base = b * ns;
+  double v = 0.0;
+  double hs = 0.0;
+  int us = 0;
+  if (act) {  // the whole group takes this branch together
+    us = bs.sh_ush[base + s];
+    const int as = bs.sh_atom[base + s];
+    hs = par.ush_hub[us];
+    double ra[3];
+    load3(pos + ((size_t)b * bs.nat_max + as) * 3, ra);
+    for (int u = g; u < n; u += PG) {
+      const int au = bs.sh_atom[base + u];
+      const double* pb = pos + ((size_t)b * bs.nat_max + au) * 3;
+      const double dx = ra[0] - pb[0], dy = ra[1] - pb[1], dz = ra[2] - pb[2];
+      const double R = sqrt(dx * dx + dy * dy + dz * dz);
+      double gm, dg;
+      es2_gamma(au == as, R, es2_avg(par.es2_average, hs, par.ush_hub[bs.sh_ush[base + u]]), gm, dg);
+      v += gm * qsh[base + u];
+    }
+  }
+  v = group_sum(v);
+  if (!live || g != 0) return;
+  if (!act) {
     vsh[t] = 0.0;
     esh[t] = 0.0;
     return;
-  }
-  const int base = b * ns;
-  const int us = bs.sh_ush[base + s];
-  const int as = bs.sh_atom[base + s];
-  const double hs = par.ush_hub[us];
-  double ra[3];
-  load3(pos + ((size_t)b * bs.nat_max + as) * 3, ra);
-  double v = 0.0;
-  const int n = nsh[b];
-  for (int u = 0; u < n; ++u) {
-    const int au = bs.sh_atom[base + u];
-    const double* pb = pos + ((size_t)b * bs.nat_max + au) * 3;
-    const double dx = ra[0] - pb[0], dy = ra[1] - pb[1], dz = ra[2] - pb[2];
-    const double R = sqrt(dx * dx + dy * dy + dz * dz);
-    double g, dg;
-    es2_gamma(au == as, R, es2_avg(par.es2_average, hs, par.ush_hub[bs.sh_ush[base + u]]), g, dg);
-    v += g * qsh[base + u];
   }
   const double q = qsh[t];
   double e = 0.5 * q * v;
@@ -184,16 +207,70 @@ __global__ void k_pot_shell(dxtb_basis_t bs, dxtb_params_t par, const int* __res
 }
 
 // ---- atom potentials: GFN1 atom ES3, GFN2 AES2 ------------------------------
-__global__ void k_pot_atom(int B, int nat_max, dxtb_params_t par, const int* __restrict__ numbers,
-                           const double* __restrict__ pos, const double* __restrict__ mrad,
-                           const double* __restrict__ qat, const double* __restrict__ qdp,
-                           const double* __restrict__ qqp, double* __restrict__ vat,
-                           double* __restrict__ vdp, double* __restrict__ vqp, double* __restrict__ eat) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= B * nat_max) return;
+__global__ __launch_bounds__(POT_BLK) void k_pot_atom(int B, int nat_max, dxtb_params_t par,
+                                                      const int* __restrict__ numbers,
+                                                      const double* __restrict__ pos, const double* __restrict__ mrad,
+                                                      const double* __restrict__ qat, const double* __restrict__ qdp,
+                                                      const double* __restrict__ qqp, double* __restrict__ vat,
+                                                      double* __restrict__ vdp, double* __restrict__ vqp,
+                                                      double* __restrict__ eat) {
+  const long tl = ((long)blockIdx.x * POT_BLK + threadIdx.x) / PG;  // target atom of this lane's group
+  const int g = threadIdx.x & (PG - 1);
+  const bool live = tl < (long)B * nat_max;
+  const int t = live ? (int)tl : 0;
   const int b = t / nat_max, i = t - b * nat_max;
-  const int z = numbers[t];
-  double va = 0.0, e = 0.0;
+  const int z = live ? numbers[t] : 0;
+  // partner sums of the AES2 potentials (GFN2): monopole, dipole (charge and
+  // dipole parts) and quadrupole potential at atom i
+  double va = 0.0, vsd[3] = {0, 0, 0}, vdd[3] = {0, 0, 0}, vsq[6] = {0, 0, 0, 0, 0, 0};
+  if (z != 0 && par.gfn == 2) {  // uniform over the group
+    double ri[3];
+    load3(pos + (size_t)t * 3, ri);
+    const double mi = mrad[t];
+    for (int j = g; j < nat_max; j += PG) {
+      const int zj = numbers[b * nat_max + j];
+      if (zj == 0 || j == i) continue;
+      const size_t tj = (size_t)b * nat_max + j;
+      const double* pj = pos + tj * 3;
+      // r = R_i - R_j (the reference's rij for the (i, j) element)
+      const double r[3] = {ri[0] - pj[0], ri[1] - pj[1], ri[2] - pj[2]};
+      const double R2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      const double R = sqrt(R2);
+      const double g1 = 1.0 / R, g3 = g1 * g1 * g1, g5 = g3 * g1 * g1;
+      const double rr = 0.5 * (mi + mrad[tj]) * g1;
+      const double f3 = 1.0 / (1.0 + 6.0 * pow(rr, par.aes_dmp3));
+      const double f5 = 1.0 / (1.0 + 6.0 * pow(rr, par.aes_dmp5));
+      const double qj = qat[tj];
+      const double muj[3] = {qdp[tj * 3], qdp[tj * 3 + 1], qdp[tj * 3 + 2]};
+      double thj[6];
+      for (int k = 0; k < 6; ++k) thj[k] = qqp[tj * 6 + k];
+      // sd[i,j] = r g3 f3 ; sd[j,i] = -sd[i,j]
+      const double s3 = g3 * f3;
+      // sq[i,j] = sq[j,i] = (x^2, 2xy, y^2, 2xz, 2yz, z^2) g5 f5
+      const double s5 = g5 * f5;
+      const double sq[6] = {r[0] * r[0] * s5, 2 * r[0] * r[1] * s5, r[1] * r[1] * s5,
+                            2 * r[0] * r[2] * s5, 2 * r[1] * r[2] * s5, r[2] * r[2] * s5};
+      // monopole potential at i: sum_j sd[i,j].mu_j + sq[i,j].th_j
+      va += s3 * (r[0] * muj[0] + r[1] * muj[1] + r[2] * muj[2]);
+      for (int k = 0; k < 6; ++k) va += sq[k] * thj[k];
+      // dipole potential at i: sum_j sd[j,i] q_j + dd[j,i] mu_j
+      for (int k = 0; k < 3; ++k) vsd[k] += -r[k] * s3 * qj;
+      const double rmu = r[0] * muj[0] + r[1] * muj[1] + r[2] * muj[2];
+      for (int k = 0; k < 3; ++k) vdd[k] += g3 * f5 * muj[k] - 3.0 * r[k] * rmu * s5;
+      // quadrupole potential at i: sum_j sq[j,i] q_j
+      for (int k = 0; k < 6; ++k) vsq[k] += sq[k] * qj;
+    }
+  }
+  va = group_sum(va);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    vsd[k] = group_sum(vsd[k]);
+    vdd[k] = group_sum(vdd[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) vsq[k] = group_sum(vsq[k]);
+  if (!live || g != 0) return;
+  double e = 0.0;
   double vd[3] = {0, 0, 0}, vq[6] = {0, 0, 0, 0, 0, 0};
   if (z != 0) {
     const double q = qat[t];
@@ -203,46 +280,9 @@ __global__ void k_pot_atom(int B, int nat_max, dxtb_params_t par, const int* __r
       e += hd * q * q * q / 3.0;
     }
     if (par.gfn == 2) {
-      double ri[3];
-      load3(pos + (size_t)t * 3, ri);
-      const double mi = mrad[t];
       double mu[3], th[6];
       for (int k = 0; k < 3; ++k) mu[k] = qdp[(size_t)t * 3 + k];
       for (int k = 0; k < 6; ++k) th[k] = qqp[(size_t)t * 6 + k];
-      double vsd[3] = {0, 0, 0}, vdd[3] = {0, 0, 0}, vsq[6] = {0, 0, 0, 0, 0, 0};
-      for (int j = 0; j < nat_max; ++j) {
-        const int zj = numbers[b * nat_max + j];
-        if (zj == 0 || j == i) continue;
-        const size_t tj = (size_t)b * nat_max + j;
-        const double* pj = pos + tj * 3;
-        // r = R_i - R_j (the reference's rij for the (i, j) element)
-        const double r[3] = {ri[0] - pj[0], ri[1] - pj[1], ri[2] - pj[2]};
-        const double R2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-        const double R = sqrt(R2);
-        const double g1 = 1.0 / R, g3 = g1 * g1 * g1, g5 = g3 * g1 * g1;
-        const double rr = 0.5 * (mi + mrad[tj]) * g1;
-        const double f3 = 1.0 / (1.0 + 6.0 * pow(rr, par.aes_dmp3));
-        const double f5 = 1.0 / (1.0 + 6.0 * pow(rr, par.aes_dmp5));
-        const double qj = qat[tj];
-        const double muj[3] = {qdp[tj * 3], qdp[tj * 3 + 1], qdp[tj * 3 + 2]};
-        double thj[6];
-        for (int k = 0; k < 6; ++k) thj[k] = qqp[tj * 6 + k];
-        // sd[i,j] = r g3 f3 ; sd[j,i] = -sd[i,j]
-        const double s3 = g3 * f3;
-        // sq[i,j] = sq[j,i] = (x^2, 2xy, y^2, 2xz, 2yz, z^2) g5 f5
-        const double s5 = g5 * f5;
-        const double sq[6] = {r[0] * r[0] * s5, 2 * r[0] * r[1] * s5, r[1] * r[1] * s5,
-                              2 * r[0] * r[2] * s5, 2 * r[1] * r[2] * s5, r[2] * r[2] * s5};
-        // monopole potential at i: sum_j sd[i,j].mu_j + sq[i,j].th_j
-        va += s3 * (r[0] * muj[0] + r[1] * muj[1] + r[2] * muj[2]);
-        for (int k = 0; k < 6; ++k) va += sq[k] * thj[k];
-        // dipole potential at i: sum_j sd[j,i] q_j + dd[j,i] mu_j
-        for (int k = 0; k < 3; ++k) vsd[k] += -r[k] * s3 * qj;
-        const double rmu = r[0] * muj[0] + r[1] * muj[1] + r[2] * muj[2];
-        for (int k = 0; k < 3; ++k) vdd[k] += g3 * f5 * muj[k] - 3.0 * r[k] * rmu * s5;
-        // quadrupole potential at i: sum_j sq[j,i] q_j
-        for (int k = 0; k < 6; ++k) vsq[k] += sq[k] * qj;
-      }
       const double dk = par.z_dkernel[z], qk = par.z_qkernel[z];
       const double scale[6] = {1, 2, 1, 2, 2, 1};
       for (int k = 0; k < 3; ++k) {
@@ -254,6 +294,8 @@ __global__ void k_pot_atom(int B, int nat_max, dxtb_params_t par, const int* __r
         e += qk * scale[k] * th[k] * th[k] + vsq[k] * th[k];
       }
     }
+  } else {
+    va = 0.0;
   }
   vat[t] = va;
   eat[t] = e;
@@ -300,10 +342,10 @@ extern "C" int dxtb_potential(const dxtb_basis_t* bas, const dxtb_params_t* par,
   if (par->gfn == 2 && (!mrad || !qdp || !qqp || !vdp || !vqp)) return DXTB_EINVAL;
   if (par->es2_gexp != 2.0) return DXTB_EUNSUP;
   const dxtb_basis_t& b = *bas;
-  hipLaunchKernelGGL(k_pot_shell, grid1((long)b.nbatch * b.nsh_max, 128), dim3(128), 0, st, b, *par, numbers,
+  hipLaunchKernelGGL(k_pot_shell, grid1((long)b.nbatch * b.nsh_max * PG, POT_BLK), dim3(POT_BLK), 0, st, b, *par,
                      pos, nsh, qsh, vsh, esh);
   DXTB_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_pot_atom, grid1((long)b.nbatch * b.nat_max, 128), dim3(128), 0, st, b.nbatch,
+  hipLaunchKernelGGL(k_pot_atom, grid1((long)b.nbatch * b.nat_max * PG, POT_BLK), dim3(POT_BLK), 0, st, b.nbatch,
                      b.nat_max, *par, numbers, pos, mrad, qat, qdp, qqp, vat, vdp, vqp, eat);
   DXTB_CHECK(hipGetLastError());
   return 0;

@@ -1,6 +1,7 @@
 // SCF step kernels: Fock build, Mulliken charges and atomic multipoles, Fermi
 // smearing.  FP64, streaming over the padded (nbatch, nao_max, nao_max)
-// matrices with one thread per element (Fock) or per AO column (charges).
+// matrices with one thread per element (Fock) or per AO column and chunk of
+// rows (charges).
 
 #include "common.h"
 #include "pairfuncs.h"
@@ -79,28 +80,42 @@ __global__ __launch_bounds__(256) void k_fock(int B, int n, const int* __restric
   F[off] = f;
 }
 
-// Per AO column j: population sum_k P_kj S_kj, band energy sum_k P_kj H0_kj and
-// the AO contributions to the atomic multipoles -sum_k P_kj M_kj (the column
-// index is the ket AO, i.e. the atom the moment is centred on).
+// Per AO column j: population sum_k P_kj S_kj, band energy sum_k P_kj H0_kj
+// and the AO contributions to the atomic multipoles -sum_k P_kj M_kj (the
+// column index is the ket AO, i.e. the atom the moment is centred on).  A
+// workgroup takes CS_COLS consecutive columns (consecutive lanes, coalesced
+// rows) and splits the rows of the molecule into CS_CHUNKS chunks, one
+// wavefront each; the chunks' sums are added in LDS.  A few molecules then
+// still spread over the device, and a lone molecule's row loop is CS_CHUNKS
+// times shorter.
+constexpr int CS_COLS = 64;
+constexpr int CS_CHUNKS = 8;
+
 template <bool MP>
-__global__ __launch_bounds__(256) void k_colsums(int B, int n, const int* __restrict__ nao,
-                                                 const double* __restrict__ P, const double* __restrict__ S,
-                                                 const double* __restrict__ H0, const double* __restrict__ D,
-                                                 const double* __restrict__ Q, double* __restrict__ out) {
-  const int b = blockIdx.y;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nb = nao[b];
-  if (j >= n) return;
+__global__ __launch_bounds__(CS_COLS * CS_CHUNKS) void k_colsums(int n, const int* __restrict__ nao,
+                                                                 const double* __restrict__ P,
+                                                                 const double* __restrict__ S,
+                                                                 const double* __restrict__ H0,
+                                                                 const double* __restrict__ D,
+                                                                 const double* __restrict__ Q,
+                                                                 double* __restrict__ out) {
   constexpr int NO = MP ? 11 : 2;
+  __shared__ double part[CS_CHUNKS - 1][NO][CS_COLS];
+  const int b = blockIdx.y;
+  const int c = threadIdx.x & (CS_COLS - 1), r = threadIdx.x / CS_COLS;
+  const int j = blockIdx.x * CS_COLS + c;
+  const int nb = nao[b];
   double acc[NO];
 #pragma unroll
   for (int t = 0; t < NO; ++t) acc[t] = 0.0;
   const size_t nn = (size_t)n * n;
+  const int rows = (nb + CS_CHUNKS - 1) / CS_CHUNKS;
+  const int k0 = r * rows, k1 = min(nb, k0 + rows);
   if (j < nb) {
     const double* Pb = P + (size_t)b * nn;
     const double* Sb = S + (size_t)b * nn;
     const double* Hb = H0 + (size_t)b * nn;
-    for (int k = 0; k < nb; ++k) {
+    for (int k = k0; k < k1; ++k) {
       const size_t e = (size_t)k * n + j;
       const double p = Pb[e];
       acc[0] += p * Sb[e];
@@ -109,11 +124,21 @@ __global__ __launch_bounds__(256) void k_colsums(int B, int n, const int* __rest
         const double* Db = D + (size_t)b * 3 * nn + e;
         const double* Qb = Q + (size_t)b * 6 * nn + e;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) acc[2 + c] -= p * Db[c * nn];
+        for (int cc = 0; cc < 3; ++cc) acc[2 + cc] -= p * Db[cc * nn];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) acc[5 + c] -= p * Qb[c * nn];
+        for (int cc = 0; cc < 6; ++cc) acc[5 + cc] -= p * Qb[cc * nn];
       }
     }
+  }
+  if (r > 0) {
+#pragma unroll
+    for (int t = 0; t < NO; ++t) part[r - 1][t][c] = acc[t];
+  }
+  __syncthreads();
+  if (r > 0 || j >= n) return;
+  for (int q = 0; q < CS_CHUNKS - 1; ++q) {
+#pragma unroll
+    for (int t = 0; t < NO; ++t) acc[t] += part[q][t][c];
   }
   double* o = out + ((size_t)b * n + j) * NO;
 #pragma unroll
@@ -162,31 +187,14 @@ __global__ void k_reduce_atoms(dxtb_basis_t bs, const int* __restrict__ nat, con
 }
 
 // ---- Fermi smearing ---------------------------------------------------------
-constexpr int FB = 256;
+// One wavefront per molecule: the sums over the orbitals are wave reductions
+// (no workgroup barriers in the root search).
+constexpr int FB = 64;
 
-__device__ double block_sum(double v, double* sh) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double r = 0.0;
-#pragma unroll
-  for (int k = 0; k < FB / 64; ++k) r += sh[k];
-  return r;
-}
-
-__device__ double block_sum_max(double v, double* sh) {
+__device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double r = sh[0];
-#pragma unroll
-  for (int k = 1; k < FB / 64; ++k) r = fmax(r, sh[k]);
-  return r;
+  return v;
 }
 
 __device__ __forceinline__ double fermi_fn(double e, double mu, double kt) {
@@ -194,64 +202,73 @@ __device__ __forceinline__ double fermi_fn(double e, double mu, double kt) {
   return x > 0.0 ? exp(-x) / (1.0 + exp(-x)) : 1.0 / (1.0 + exp(x));
 }
 
-// One block per molecule.  For every spin channel the chemical potential is
-// the root of N(mu) = sum_i f_i(mu) - n (monotonic): bracketed bisection
-// until the bracket is within a few kT, then safeguarded Newton steps to
-// machine precision (the converged result of filling.py:428-742).
+// For every spin channel the chemical potential is the root of
+// N(mu) = sum_i f_i(mu) - n (monotonic): bracketed bisection until the
+// bracket is within a few kT, then safeguarded Newton steps to machine
+// precision (the converged result of filling.py:428-742).  Channels with the
+// same electron count (closed shells) share one search.
 __global__ __launch_bounds__(FB) void k_fermi(int n, const int* __restrict__ nao, const double* __restrict__ emo,
                                               const double* __restrict__ nab, double kt,
                                               double* __restrict__ occ, double* __restrict__ ts) {
-  __shared__ double sh[FB / 64];
   const int b = blockIdx.x;
+  const int lane = threadIdx.x;
   const int nb = nao[b];
   const double* e = emo + (size_t)b * n;
+  // bracket from the extreme levels (the eigensolver need not sort them)
+  double emin = 1e300, emax = -1e300;
+  for (int i = lane; i < nb; i += FB) {
+    emin = fmin(emin, e[i]);
+    emax = fmax(emax, e[i]);
+  }
+  emin = -wave_max(-emin);
+  emax = wave_max(emax);
+  const bool same = nab[b * 2] == nab[b * 2 + 1];
   double ent = 0.0;
-  for (int spin = 0; spin < 2; ++spin) {
+  for (int spin = 0; spin < (same ? 1 : 2); ++spin) {
     const double target = nab[b * 2 + spin];
     double* o = occ + ((size_t)b * 2 + spin) * n;
+    double* o2 = same ? occ + ((size_t)b * 2 + 1) * n : o;
     if (target <= 1e-12 || nb == 0) {
-      for (int i = threadIdx.x; i < n; i += FB) o[i] = 0.0;
+      for (int i = lane; i < n; i += FB) {
+        o[i] = 0.0;
+        o2[i] = 0.0;
+      }
       continue;
     }
-    // bracket from the extreme levels (the eigensolver need not sort them)
-    double emin = 1e300, emax = -1e300;
-    for (int i = threadIdx.x; i < nb; i += FB) {
-      emin = fmin(emin, e[i]);
-      emax = fmax(emax, e[i]);
-    }
-    emin = -block_sum_max(-emin, sh);
-    emax = block_sum_max(emax, sh);
     double lo = emin - 1.0, hi = emax + 1.0;
     double mu = 0.5 * (lo + hi);
     for (int it = 0; it < 200; ++it) {
-      double s = 0.0, ds = 0.0;
-      for (int i = threadIdx.x; i < nb; i += FB) {
+      double sm = 0.0, ds = 0.0;
+      for (int i = lane; i < nb; i += FB) {
         const double f = fermi_fn(e[i], mu, kt);
-        s += f;
+        sm += f;
         ds += f * (1.0 - f);
       }
-      s = block_sum(s, sh) - target;
-      ds = block_sum(ds, sh) / kt;
-      if (s > 0.0) hi = mu; else lo = mu;
-      if (fabs(s) < 1e-13 * (1.0 + target)) break;
-      double nm = mu - s / ds;
+      sm = wave_sum(sm) - target;
+      ds = wave_sum(ds) / kt;
+      if (sm > 0.0) hi = mu; else lo = mu;
+      if (fabs(sm) < 1e-13 * (1.0 + target)) break;
+      double nm = mu - sm / ds;
       if (!(ds > 0.0) || !(nm > lo && nm < hi) || (hi - lo) > 50.0 * kt) nm = 0.5 * (lo + hi);
       if (nm == mu) break;
       mu = nm;
     }
-    for (int i = threadIdx.x; i < n; i += FB) {
+    double es = 0.0;
+    for (int i = lane; i < n; i += FB) {
       double f = 0.0;
       if (i < nb) {
         f = fermi_fn(e[i], mu, kt);
         // -TS: f ln f + (1 - f) ln(1 - f) with clamps at eps (scf/base.py:554-619)
         const double a = fmax(f, dxtb::EPS), c = fmax(1.0 - f, dxtb::EPS);
-        ent += a * log(a) + c * log(c);
+        es += a * log(a) + c * log(c);
       }
       o[i] = f;
+      o2[i] = f;
     }
+    ent += same ? 2.0 * es : es;
   }
-  ent = block_sum(ent, sh);
-  if (threadIdx.x == 0) ts[b] = ent * kt;
+  ent = wave_sum(ent);
+  if (lane == 0) ts[b] = ent * kt;
 }
 
 }  // namespace
@@ -274,6 +291,11 @@ extern "C" int dxtb_fock(const dxtb_basis_t* bas, const int* nao, const int* ao_
   return (int)hipGetLastError();
 }
 
+extern "C" size_t dxtb_charges_work_size(int nbatch, int nao_max) {
+  if (nbatch <= 0 || nao_max <= 0) return 0;
+  return (size_t)nbatch * nao_max * 11 * sizeof(double);
+}
+
 extern "C" int dxtb_charges(const dxtb_basis_t* bas, const int* nao, const int* nat, const int* at_sh0,
                             const int* at_nsh, const double* n0, const double* P, const double* S,
                             const double* H0, const double* D, const double* Q, double* qsh, double* qat,
@@ -282,16 +304,19 @@ extern "C" int dxtb_charges(const dxtb_basis_t* bas, const int* nao, const int* 
     return DXTB_EINVAL;
   const bool mp = D != nullptr;
   if (mp && (!Q || !qdp || !qqp)) return DXTB_EINVAL;
+  if (bas->nbatch <= 0) return DXTB_OK;
+  if (bas->nbatch > 65535) return DXTB_EINVAL;  // grid y
   const int n = bas->nao_max;
-  const dim3 g((unsigned)((n + 127) / 128), (unsigned)bas->nbatch);
+  const dim3 g((unsigned)((n + CS_COLS - 1) / CS_COLS), (unsigned)bas->nbatch);
+  const dim3 blk(CS_COLS * CS_CHUNKS);
   const dim3 ga = grid1((long)bas->nbatch * bas->nat_max, 128);
   if (mp) {
-    hipLaunchKernelGGL(k_colsums<true>, g, dim3(128), 0, st, bas->nbatch, n, nao, P, S, H0, D, Q, work);
+    hipLaunchKernelGGL(k_colsums<true>, g, blk, 0, st, n, nao, P, S, H0, D, Q, work);
     DXTB_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_reduce_atoms<true>, ga, dim3(128), 0, st, *bas, nat, at_sh0, at_nsh, n0, work, qsh, qat,
                        qdp, qqp, eh0);
   } else {
-    hipLaunchKernelGGL(k_colsums<false>, g, dim3(128), 0, st, bas->nbatch, n, nao, P, S, H0, D, Q, work);
+    hipLaunchKernelGGL(k_colsums<false>, g, blk, 0, st, n, nao, P, S, H0, D, Q, work);
     DXTB_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_reduce_atoms<false>, ga, dim3(128), 0, st, *bas, nat, at_sh0, at_nsh, n0, work, qsh,
                        qat, qdp, qqp, eh0);

@@ -608,7 +608,7 @@ class GeometryState:
         eh0 = torch.empty((B, na), **f64)
         qdp = torch.empty((B, na, 3), **f64) if self.mp else None
         qqp = torch.empty((B, na, 6), **f64) if self.mp else None
-        work = torch.empty((B, n, 11), **f64)
+        work = torch.empty(max(self.lib.dxtb_charges_work_size(B, n) // 8, 1), **f64)
         rc = self.lib.dxtb_charges(self.bs, _lib.ptr(d["nao"]), _lib.ptr(d["nat"]), _lib.ptr(d["at_sh0"]),
                                    _lib.ptr(d["at_nsh"]), _lib.ptr(self.n0), _lib.ptr(P), _lib.ptr(self.S),
                                    _lib.ptr(self.H0), _lib.ptr(self.D), _lib.ptr(self.Q), _lib.ptr(qsh),
@@ -687,7 +687,8 @@ class DeviceBroyden:
         self.ctl = torch.zeros((B, 4), dtype=torch.int32, device=device)
         self.flag = torch.zeros(B, dtype=torch.int32, device=device)
         self.dxn = torch.zeros(B, **f64)
-        self.work = torch.empty(B * 3 * L, **f64)
+        self.wstride = _lib.load().dxtb_broyden_work_size(1, L, H) // 8  # doubles per molecule
+        self.work = torch.empty(B * self.wstride, **f64)
 
     _ROWS = ("xe", "x", "y", "hc", "hd", "ctl", "flag", "dxn")
 
@@ -706,7 +707,7 @@ class DeviceBroyden:
     def subset(self, keep: torch.Tensor) -> None:
         for name in self._ROWS:
             setattr(self, name, getattr(self, name).index_select(0, keep).contiguous())
-        self.work = self.work[: keep.numel() * 3 * self.L]
+        self.work = self.work[: keep.numel() * self.wstride]
 
     def move(self, src: torch.Tensor, dst: torch.Tensor) -> None:
         """Copy rows src into rows dst (disjoint), in place."""
@@ -714,14 +715,18 @@ class DeviceBroyden:
             t = getattr(self, name)
             t.index_copy_(0, dst, t.index_select(0, src))
 
-    def step(self, gx: torch.Tensor, w: torch.Tensor, o: "SCFOptions", nb: int | None = None) -> None:
-        """One iteration of the first ``nb`` rows (default: all) with g(xe) = gx."""
+    def step(self, gx: torch.Tensor, w: torch.Tensor, o: "SCFOptions", nb: int | None = None,
+             kbound: int = 0) -> None:
+        """One iteration of the first ``nb`` rows (default: all) with g(xe) = gx.
+        ``kbound`` (0: the history capacity) bounds the rows' history lengths
+        (e.g. the iterations they have run); it only sizes the launch."""
         nb = self.xe.shape[0] if nb is None else nb
         lib = _lib.load()
         rc = lib.dxtb_broyden_step(nb, self.L, self.H, _lib.ptr(w), _lib.ptr(gx.contiguous()), _lib.ptr(self.xe),
                                    _lib.ptr(self.x), _lib.ptr(self.y), _lib.ptr(self.hc), _lib.ptr(self.hd),
                                    _lib.ptr(self.ctl), _lib.ptr(self.flag), _lib.ptr(self.dxn), _lib.ptr(self.work),
-                                   o.alpha, o.damp, o.x_tol, o.f_tol, o.maxiter, _lib.stream_ptr(self.xe.device))
+                                   o.alpha, o.damp, o.x_tol, o.f_tol, o.maxiter, int(kbound),
+                                   _lib.stream_ptr(self.xe.device))
         _lib.check(rc, "dxtb_broyden_step")
 
 
@@ -853,7 +858,7 @@ def run_scf(state: GeometryState, cull: float = 0.75) -> SCFResult:
         if o.adaptive_eigensolver:
             started = mix.ctl[:, 1] != 0
             cur._jtol = torch.where(started, jacobi_tolerance(wnorm(mix.y, cur.wvec)), JACOBI_TOL_LOOSE).contiguous()
-        mix.step(cur.g(mix.xe), cur.wvec, o)
+        mix.step(cur.g(mix.xe), cur.wvec, o, kbound=it + 1)  # a history grows by one entry per iteration
         nb = cur.bb.nbatch
         hflags[it % 2][:nb].copy_(mix.flag, non_blocking=True)
         events[it % 2] = torch.cuda.Event()

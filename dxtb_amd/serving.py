@@ -147,6 +147,10 @@ class SlotPool:
         self.mix = hscf.DeviceBroyden(capacity, L, opts.maxiter + 1, self.dev)
         self.occupied = np.zeros(capacity, dtype=bool)
         self.req: list[Request | None] = [None] * capacity
+        # pool iterations launched, and the count at each slot's admission: a
+        # slot's Broyden history holds at most the iterations since then
+        self.iters_run = 0
+        self.adm_it = np.zeros(capacity, dtype=np.int64)
         # pinned staging of the stop flags and iteration counts, reused every
         # iteration: an iteration is launched only after the previous one (and
         # so its copies) completed
@@ -201,6 +205,7 @@ class SlotPool:
         self.state.scatter(rows64, st)
         self.mix.start(rt)
         self.occupied[rows] = True
+        self.adm_it[rows] = self.iters_run
         for r, row in zip(reqs, rows):
             r.admitted = now
             self.req[row] = r
@@ -223,6 +228,7 @@ class SlotPool:
         self.mix.ctl[:, 0].index_fill_(0, st, 0)  # the source slots are free
         self.occupied[dst] = True
         self.occupied[src] = False
+        self.adm_it[dst] = self.adm_it[src]
         for s_, d_ in zip(src, dst):
             self.req[d_], self.req[s_] = self.req[s_], None
         self.moves += len(src)
@@ -242,10 +248,11 @@ class SlotPool:
                 return w
         return self.cap
 
-    def _iterate(self, W: int) -> None:
+    def _iterate(self, W: int, kbound: int = 0) -> None:
         """One iteration over rows [0, W): the SCF map at every row's next
         point, then the device Broyden step (dxtb_broyden_step), which skips
-        free and stopped slots.  No host data, fixed shapes (capturable)."""
+        free and stopped slots.  No host data, fixed shapes (capturable);
+        ``kbound`` bounds the rows' history lengths (0: the capacity)."""
         o = self.opts
         win = self._wins.get(W)
         if win is None:
@@ -259,7 +266,7 @@ class SlotPool:
             win._jtol = torch.where(started, hscf.jacobi_tolerance(hscf.wnorm(m.y[:W], win.wvec)),
                                     hscf.JACOBI_TOL_LOOSE).contiguous()
         gx = win.g(m.xe[:W])
-        m.step(gx, win.wvec, o, nb=W)
+        m.step(gx, win.wvec, o, nb=W, kbound=kbound)
 
     def launch(self):
         """Enqueue one iteration over the occupied window on the current
@@ -282,7 +289,9 @@ class SlotPool:
             else:
                 g.replay()
         else:
-            self._iterate(W)
+            live = self.occupied[:W]
+            self._iterate(W, kbound=int(self.iters_run - self.adm_it[:W][live].min()) + 1)
+        self.iters_run += 1
         # read by finish() once the pool's event completed
         self.h_flag[:hi].copy_(self.mix.flag[:hi], non_blocking=True)
         self.h_iter[:hi].copy_(self.mix.ctl[:hi, 3], non_blocking=True)

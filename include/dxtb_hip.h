@@ -140,8 +140,10 @@ int dxtb_fock(const dxtb_basis_t* bas, const int* nao, const int* ao_shell, cons
  * qdp (nbatch, nat_max, 3), qqp (nbatch, nat_max, 6) and the per-atom
  * band energy eh0 (nbatch, nat_max) = sum diag(P H0).  n0: (nbatch, nao_max)
  * reference occupations.  at_sh0/at_nsh: first shell / shell count per atom.
- * work: (nbatch, nao_max, 11) scratch.
+ * work: dxtb_charges_work_size(nbatch, nao_max) bytes of scratch (the
+ * column sums).
  * Replaces scf/pure/conversions.py:203-252 (density_to_charges). */
+size_t dxtb_charges_work_size(int nbatch, int nao_max);
 int dxtb_charges(const dxtb_basis_t* bas, const int* nao, const int* nat, const int* at_sh0, const int* at_nsh,
                  const double* n0, const double* P, const double* S, const double* H0, const double* D,
                  const double* Q, double* qsh, double* qat, double* qdp, double* qqp, double* eh0,
@@ -260,8 +262,10 @@ int dxtb_gemm_batched(int nbatch, int n, const int* nsize, int transa, int trans
                       const double* B, const double* scale, double beta, const double* Cin, double* Out, int sym,
                       int fill_pad, double pad, hipStream_t stream);
 
-/* Bytes of the workspace of dxtb_broyden_step (nbatch x 3 x L doubles). */
-size_t dxtb_broyden_work_size(int nbatch, int L);
+/* Bytes of the workspace of dxtb_broyden_step for histories of H entries
+ * (per molecule 3 H coefficients and the partial combinations,
+ * ceil(H / 64) x 3 x L doubles). */
+size_t dxtb_broyden_work_size(int nbatch, int L, int H);
 
 /* One iteration of the per-molecule Broyden mixer of the SCF, after the
  * caller has evaluated the SCF map at xe (gx = g(xe)), for nbatch molecules
@@ -276,14 +280,17 @@ size_t dxtb_broyden_work_size(int nbatch, int L);
  * ctl (nbatch, 4) int32: occupied, started, history length, iterations;
  * a molecule with occupied == 0 is skipped; one that converges (flag 1) or
  * reaches maxiter iterations (flag 2) gets occupied = 0.  dxn (nbatch): norm
- * of the last step.  work: dxtb_broyden_work_size bytes.  H <= 4096.
+ * of the last step.  work: dxtb_broyden_work_size(nbatch, L, H) bytes.
+ * H <= 4096, nbatch <= 65535.  kbound (<= 0: H) is an upper bound the caller
+ * knows for the history lengths; it sizes the launch grids only (every
+ * history length gives the same result).
  * Replaces the broyden1 root finder of the reference's implicit SCF
  * (exlibs/xitorch/_impls/optimize/root/rootsolver.py:42-261 with
  * _jacobian.py:51-120, 250-316; alpha = -0.5, damp = 0.5 as set in
  * scf/base.py:268-287), applied per molecule. */
 int dxtb_broyden_step(int nbatch, int L, int H, const double* w, const double* gx, double* xe, double* x, double* y,
                       double* hc, double* hd, int* ctl, int* flag, double* dxn, double* work, double alpha,
-                      double damp, double x_tol, double f_tol, int maxiter, hipStream_t stream);
+                      double damp, double x_tol, double f_tol, int maxiter, int kbound, hipStream_t stream);
 
 /* Largest matrix order handled by dxtb_jacobi_eigh. */
 int dxtb_jacobi_max_n(void);

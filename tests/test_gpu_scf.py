@@ -198,10 +198,10 @@ def test_forces_central_differences_config2(cuda):
     picks = rng.choice(200, size=24, replace=False)
     h = 1e-4
     # small-gap molecules are ill-conditioned: a displaced copy of one that
-    # converges in 182 iterations can need over 250, so this test allows 500
-    # (the fixed point does not depend on the limit) and keeps the slow
-    # molecules that set the batch's tail
-    opts = hscf.SCFOptions(maxiter=500)
+    # converges in under 200 iterations can need 300-600 (the count follows
+    # rounding), so this test allows 1000 (the fixed point does not depend on
+    # the limit) and keeps the slow molecules that set the batch's tail
+    opts = hscf.SCFOptions(maxiter=1000)
     N, P = workloads.pad([nums[i] for i in picks], [poss[i] for i in picks])
     bb = hbasis.build_batch(N, "gfn2", cuda)
     res = hscf.run_scf(hscf.GeometryState(bb, torch.as_tensor(P, device=cuda), opts=opts))
