@@ -24,6 +24,8 @@ vector; converged molecules are frozen while the rest keeps iterating.
 
 from __future__ import annotations
 
+import collections
+
 import ctypes
 from dataclasses import dataclass
 
@@ -128,6 +130,7 @@ class _SolveHistory:
 
     FRESH = 12
     HISTORY = 8
+    PENDING = 8  # sweep counts in flight (their copies to the host not yet complete)
 
     def __init__(self, n: int = 0):
         # a missing split sweep costs a one-workgroup sweep: 20-140 ms above
@@ -137,20 +140,27 @@ class _SolveHistory:
         self.recent: dict[int, list[int]] = {}  # size class -> last counts
         self.bump = False
         self.calls = 0
-        self.sw = self.host = self.ev = None
-        self.n = 0
-        self.ev_class = 0
+        self.sw = None
+        # counts copied back without blocking: (event, pinned buffer, rows,
+        # size class) in launch order; a solve's count is used once its copy
+        # has completed, however far the host runs ahead of the device
+        self.pending: collections.deque = collections.deque()
+        self.free: list = []  # pinned buffers not in flight
 
     @staticmethod
     def size_class(nb: int) -> int:
         return 0 if nb <= 8 else (1 if nb <= 64 else 2)
 
+    def _drain(self) -> None:
+        while self.pending and self.pending[0][0].query():
+            _, buf, n, cls = self.pending.popleft()
+            h = self.recent.get(cls, [])
+            self.recent[cls] = (h + [int(buf[:n].max())])[-self.HISTORY:]
+            self.free.append(buf)
+
     def predict(self, nb: int) -> int:
-        capturing = torch.cuda.is_current_stream_capturing()  # no event queries inside a capture
-        if self.ev is not None and not capturing and self.ev.query():
-            h = self.recent.get(self.ev_class, [])
-            self.recent[self.ev_class] = (h + [int(self.host[: self.n].max())])[-self.HISTORY:]
-            self.ev = None
+        if not torch.cuda.is_current_stream_capturing():  # no event queries inside a capture
+            self._drain()
         hist = self.recent.get(self.size_class(nb)) or [v for h in self.recent.values() for v in h]
         pred = max(hist) + self.margin if hist else self.fresh_sweeps
         if self.bump:
@@ -170,18 +180,22 @@ class _SolveHistory:
     def counts(self, nb: int, dev) -> torch.Tensor:
         if self.sw is None or self.sw.numel() < nb:
             self.sw = torch.zeros(nb, dtype=torch.int32, device=dev)
-            self.host = torch.zeros(nb, dtype=torch.int32, pin_memory=True)
-            self.ev = None
         return self.sw[:nb]
 
     def record(self, nb: int, dev) -> None:
         if torch.cuda.is_current_stream_capturing():
             return
-        self.host[:nb].copy_(self.sw[:nb], non_blocking=True)
-        self.ev = torch.cuda.Event()
-        self.ev.record(torch.cuda.current_stream(dev))
-        self.n = nb
-        self.ev_class = self.size_class(nb)
+        if len(self.pending) >= self.PENDING:
+            return  # the host is far ahead: this count is not kept
+        buf = next((b for b in self.free if b.numel() >= nb), None)
+        if buf is None:
+            buf = torch.zeros(max(nb, 8), dtype=torch.int32, pin_memory=True)
+        else:
+            self.free.remove(buf)
+        buf[:nb].copy_(self.sw[:nb], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(dev))
+        self.pending.append((ev, buf, nb, self.size_class(nb)))
 
 
 # eigensolves of the overlap matrix (orthogonaliser): from the unit matrix,
