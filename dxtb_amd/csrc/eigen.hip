@@ -92,8 +92,8 @@ __device__ __forceinline__ bool rotation(double app, double aqq, double apq, dou
   // t = sign(tau) / (|tau| + sqrt(1 + tau^2)) with numerator and denominator
   // multiplied by 2|a_pq|: one square root and one division
   const double d = aqq - app;
-  const double t = (d >= 0.0 ? 2.0 * apq : -2.0 * apq) / (fabs(d) + sqrt(d * d + 4.0 * apq * apq));
-  c = rsqrt(1.0 + t * t);
+  const double t = (d >= 0.0 ? 2.0 * apq : -2.0 * apq) / (fabs(d) + sqrt(fma(d, d, 4.0 * apq * apq)));
+  c = rsqrt(fma(t, t, 1.0));
   sn = t * c;
   return true;
 }
@@ -329,6 +329,27 @@ __device__ __forceinline__ double row_share(double v) { return dpp_f64<0x150 + L
 // Value of lane x + 1 (mod 16) of this lane's row of 16 lanes (DPP
 // row_ror:15; checked on the MI355X: lane x of every row receives lane x + 1).
 __device__ __forceinline__ double row_next(double v) { return dpp_f64<0x12F>(v); }
+
+// The 2x2 tile update of a cross sweep, tile <- R_j^T tile R_x with
+// R = (c, s; -s, c), and the update of a Q row pair, (a, b) <- (a, b) R_x.
+// Written with explicit fma so that the one-wavefront and the four-wavefront
+// forms round identically: the split and one-workgroup solvers then run the
+// same arithmetic and agree bit for bit.
+__device__ __forceinline__ void tile_rot(double c1, double s1, double c2, double s2, double& aii, double& bij,
+                                         double& bji, double& bjj) {
+  const double lpp = fma(c1, aii, -(s1 * bji)), lpq = fma(c1, bij, -(s1 * bjj));
+  const double lqp = fma(s1, aii, c1 * bji), lqq = fma(s1, bij, c1 * bjj);
+  aii = fma(c2, lpp, -(s2 * lpq));
+  bij = fma(s2, lpp, c2 * lpq);
+  bji = fma(c2, lqp, -(s2 * lqq));
+  bjj = fma(s2, lqp, c2 * lqq);
+}
+
+__device__ __forceinline__ void qrow_rot(double c2, double s2, double& a, double& b) {
+  const double a0 = a, b0 = b;
+  a = fma(c2, a0, -(s2 * b0));
+  b = fma(s2, a0, c2 * b0);
+}
 
 // One sweep of the 32x32 subproblem in W by one wavefront.  Lane j of row r
 // (lane = 16 r + j) computes the rotation of pair kk = (j + r) mod 16 and owns
@@ -607,19 +628,10 @@ __device__ bool bj_inner_cross(WaveLds& W, const BjMat& M, int k, int s, double 
 #pragma unroll
       for (int u = 0; u < 4; ++u) {  // tile (4 g + u, x) <- R_j^T tile R_x
         const double c1 = cs[2 * (4 * g + u)], s1 = cs[2 * (4 * g + u) + 1];
-        const double lpp = c1 * aii[u] - s1 * bji[u], lpq = c1 * bij[u] - s1 * bjj[u];
-        const double lqp = s1 * aii[u] + c1 * bji[u], lqq = s1 * bij[u] + c1 * bjj[u];
-        aii[u] = c2 * lpp - s2 * lpq;
-        bij[u] = s2 * lpp + c2 * lpq;
-        bji[u] = c2 * lqp - s2 * lqq;
-        bjj[u] = s2 * lqp + c2 * lqq;
+        tile_rot(c1, s1, c2, s2, aii[u], bij[u], bji[u], bjj[u]);
       }
 #pragma unroll
-      for (int v = 0; v < 8; ++v) {  // Q <- Q J: columns I_x and J_sigma(x)
-        const double a = qi[v], b = qj[v];
-        qi[v] = c2 * a - s2 * b;
-        qj[v] = s2 * a + c2 * b;
-      }
+      for (int v = 0; v < 8; ++v) qrow_rot(c2, s2, qi[v], qj[v]);  // Q <- Q J: columns I_x and J_sigma(x)
       wave_sync();  // the coefficients are read before the next round writes them
     }
     // advance sigma: J-side columns take slot x + 1 (a rotation within the
@@ -658,6 +670,98 @@ __device__ bool bj_inner_cross(WaveLds& W, const BjMat& M, int k, int s, double 
     }
   }
   wave_sync();
+  return any;
+}
+
+// The cross sweep of bj_inner_cross spread over CW = 4 wavefronts (split
+// form, outer steps s > 0).  Same slot space, rotations and update formulas;
+// lane l of wavefront w owns the single tile at row slot j = 4 (l / 16) + w,
+// column slot x = l % 16 of the four 16x16 blocks, and the Q rows
+// i = 8 (l / 16) + 2 w + v (v < 2).  Per round: the 16 lanes holding a
+// diagonal tile (j = x) compute the rotations, which all wavefronts read from
+// LDS after a barrier; the J-side column advance stays within each row of 16
+// lanes (DPP), the J-side row advance (slot j takes j + 1, which lives in the
+// next wavefront) goes through LDS.  Both LDS exchanges are double buffered by
+// round parity, so a round needs two barriers.
+constexpr int CW = 4;
+
+struct CrossLds {
+  double cs[2][2 * BS];          // (c, s) of the 16 pairs, by round parity
+  double xb[2][2][BS][BS + 1];   // bji and bjj rows for the row advance, by round parity
+  int vote[2][CW];               // per wavefront: rotated anything this round
+};
+
+__device__ bool bj_inner_cross4(CrossLds& X, const BjMat& M, int k, int s, double thr, int tid) {
+  int bi, bj;
+  block_pair(k, s, M.nb, bi, bj);
+  const int w = tid >> 6, lane = tid & 63;
+  const int x = lane & 15, g = lane >> 4;
+  const int j = 4 * g + w;  // row slot
+  const int n = M.n, nmax = M.nmax, last = M.nlast;
+  const int gix = bi * BS + x, gjx = bj * BS + x;
+  const int gij = bi * BS + j, gjj = bj * BS + j;
+  double aii, bij, bji, bjj, qi[2], qj[2];
+  {
+    const double* Ai = M.A + (size_t)min(gij, last) * nmax;
+    const double* Aj = M.A + (size_t)min(gjj, last) * nmax;
+    aii = Ai[min(gix, last)] * mask01(gij < n && gix < n);
+    bij = Ai[min(gjx, last)] * mask01(gij < n && gjx < n);
+    bji = Aj[min(gix, last)] * mask01(gjj < n && gix < n);
+    bjj = Aj[min(gjx, last)] * mask01(gjj < n && gjx < n);
+  }
+#pragma unroll
+  for (int v = 0; v < 2; ++v) {
+    const int i = 8 * g + 2 * w + v;
+    qi[v] = (i == x) ? 1.0 : 0.0;
+    qj[v] = (i == BS + x) ? 1.0 : 0.0;
+  }
+  const bool diag = j == x;
+  const int jn = (j + 1) & (BS - 1), xn = (x + 1) & (BS - 1);
+  bool any = false;
+  for (int r = 0; r < BS; ++r) {
+    const int par = r & 1;
+    double c = 1.0, sn = 0.0;
+    const bool rot = diag && rotation(aii, bjj, bij, thr, c, sn);
+    if (diag) {
+      X.cs[par][2 * x] = c;
+      X.cs[par][2 * x + 1] = sn;
+    }
+    const bool wrot = __any(rot ? 1 : 0);
+    if (lane == 0) X.vote[par][w] = wrot ? 1 : 0;
+    __syncthreads();
+    const bool arot = (X.vote[par][0] | X.vote[par][1] | X.vote[par][2] | X.vote[par][3]) != 0;
+    if (arot) {  // uniform over the workgroup
+      any = true;
+      const double c1 = X.cs[par][2 * j], s1 = X.cs[par][2 * j + 1];
+      const double c2 = X.cs[par][2 * x], s2 = X.cs[par][2 * x + 1];
+      tile_rot(c1, s1, c2, s2, aii, bij, bji, bjj);  // tile (j, x) <- R_j^T tile R_x
+#pragma unroll
+      for (int v = 0; v < 2; ++v) qrow_rot(c2, s2, qi[v], qj[v]);  // Q <- Q J: columns I_x and J_sigma(x)
+    }
+    // advance sigma: J-side columns take slot x + 1 (DPP within the row of 16
+    // lanes), J-side rows slot j + 1 (through LDS)
+    bij = row_next(bij);
+#pragma unroll
+    for (int v = 0; v < 2; ++v) qj[v] = row_next(qj[v]);
+    X.xb[par][0][j][x] = bji;
+    X.xb[par][1][j][x] = bjj;
+    __syncthreads();
+    bji = X.xb[par][0][jn][x];
+    bjj = X.xb[par][1][jn][xn];
+  }
+  if (any) {  // uniform over the workgroup
+    if (gij < n && gix < n) M.A[(size_t)gij * nmax + gix] = aii;
+    if (gij < n && gjx < n) M.A[(size_t)gij * nmax + gjx] = bij;
+    if (gjj < n && gix < n) M.A[(size_t)gjj * nmax + gix] = bji;
+    if (gjj < n && gjx < n) M.A[(size_t)gjj * nmax + gjx] = bjj;
+    double* Q = qstep(M, s) + (size_t)k * SB * SB;
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int i = 8 * g + 2 * w + v;
+      Q[i * SB + x] = qi[v];
+      Q[i * SB + BS + x] = qj[v];
+    }
+  }
   return any;
 }
 
@@ -1005,6 +1109,21 @@ __global__ __launch_bounds__(64) void k_bj_inner(int nmax, const JArgs* __restri
   if (threadIdx.x == 0) fstep(M, s)[k] = any ? 1 : 0;
 }
 
+// Inner phase of the split form at outer steps s > 0: one CW-wavefront
+// workgroup per (block pair, matrix).
+__global__ __launch_bounds__(CW * 64) void k_bj_inner_cross(int nmax, const JArgs* __restrict__ ja,
+                                                           double* __restrict__ work, const double* __restrict__ sst,
+                                                           int* __restrict__ rot, int s) {
+  __shared__ CrossLds X;
+  const int b = blockIdx.y, k = blockIdx.x;
+  const JArgsC jc = jconst(ja);
+  if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
+  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, rot, jc->defer);
+  if (k >= M.npairs || s >= M.nb - 1) return;
+  const bool any = bj_inner_cross4(X, M, k, s, sst[4 * b + 1], threadIdx.x);
+  if (threadIdx.x == 0) fstep(M, s)[k] = any ? 1 : 0;
+}
+
 __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(5))) void k_bj_product(int nmax, const JArgs* __restrict__ ja, double* __restrict__ work,
                                                    const double* __restrict__ sst, int* __restrict__ rot,
                                                    int s) {
@@ -1095,8 +1214,12 @@ hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax,
     for (int sw = 0; sw < nsw; ++sw) {
       hipLaunchKernelGGL(k_bj_check, dim3((unsigned)nbcap), dim3(JBB), 0, st, nmax, w.ja, w.rot_q, tol, w.sst, sw);
       for (int s = 0; s < nb - 1; ++s) {
-        hipLaunchKernelGGL(k_bj_inner, dim3((unsigned)npairs, (unsigned)nbcap), dim3(64), 0, st, nmax, w.ja,
-                           w.rot_q, w.sst, w.rot, s);
+        if (s == 0)  // the full sweep of every subproblem (from LDS), then the cross sweeps
+          hipLaunchKernelGGL(k_bj_inner, dim3((unsigned)npairs, (unsigned)nbcap), dim3(64), 0, st, nmax, w.ja,
+                             w.rot_q, w.sst, w.rot, s);
+        else
+          hipLaunchKernelGGL(k_bj_inner_cross, dim3((unsigned)npairs, (unsigned)nbcap), dim3(CW * 64), 0, st, nmax,
+                             w.ja, w.rot_q, w.sst, w.rot, s);
         if (nwork > 0)  // one block pair with deferred VT: nothing to multiply
           hipLaunchKernelGGL(k_bj_product, dim3((unsigned)nbcap, (unsigned)((nwork + PW - 1) / PW)), dim3(PW * 64),
                              PROD_LDS, st, nmax, w.ja, w.rot_q, w.sst, w.rot, s);

@@ -23,6 +23,7 @@ window of rows up to its highest occupied slot.
 
 from __future__ import annotations
 
+import collections
 import time
 from dataclasses import dataclass, field
 
@@ -34,6 +35,10 @@ from .basis import build_batch, shell_table
 
 # static window sizes of the captured iteration graphs
 GRAPH_WINDOWS = (8, 16, 32, 64, 128, 256)
+# iterations a pool keeps in flight: the next one is already on the pool's
+# stream while the host reads the flags of the last one and retires and
+# admits molecules, so the stream does not wait for the host
+PIPELINE = 2
 # basis-size classes (padded nao) of the slot pools
 POOL_NAO = (48, 96, 144, 192, 256, 320, 384, 448, 512, 640, 768, 1024, 1280, 1536)
 # padded atom and shell counts of a class, as fractions of its nao; a molecule
@@ -151,11 +156,10 @@ class SlotPool:
         # slot's Broyden history holds at most the iterations since then
         self.iters_run = 0
         self.adm_it = np.zeros(capacity, dtype=np.int64)
-        # pinned staging of the stop flags and iteration counts, reused every
-        # iteration: an iteration is launched only after the previous one (and
-        # so its copies) completed
-        self.h_flag = torch.zeros(capacity, dtype=torch.int32, pin_memory=True)
-        self.h_iter = torch.zeros(capacity, dtype=torch.int32, pin_memory=True)
+        # pinned staging of the stop flags and iteration counts, one pair per
+        # iteration in flight (PIPELINE of them) plus one
+        self._stage = [(torch.zeros(capacity, dtype=torch.int32, pin_memory=True),
+                        torch.zeros(capacity, dtype=torch.int32, pin_memory=True)) for _ in range(PIPELINE + 1)]
         self._wins: dict = {}
         # graph mode needs a capturable eigensolver (the Jacobi kernels)
         self.graphs = graphs and nao_max <= self.state.lib.dxtb_jacobi_max_n()
@@ -163,8 +167,8 @@ class SlotPool:
         self.graph_wmax = capacity if graph_window_max is None else graph_window_max
         self._graphs: dict = {}
         self._gpool = torch.cuda.graph_pool_handle() if self.graphs else None
-        self.pending = None  # launched iteration awaiting finish()
-        self.event = torch.cuda.Event()
+        self.inflight: collections.deque = collections.deque()  # launched iterations awaiting finish()
+        self.want_compact = False  # compact once no iteration is in flight
         self.retiring: list = []  # (event, requests, host energy, host forces, ...) in flight
         self.moves = 0  # slots moved by compaction
         self.admitted_busy = 0  # requests admitted while other slots were mid-SCF
@@ -219,6 +223,7 @@ class SlotPool:
         occ = np.nonzero(self.occupied)[0]
         n = len(occ)
         if n == 0 or occ[-1] < n:
+            self.want_compact = False
             return False
         src = occ[occ >= n]
         dst = np.nonzero(~self.occupied[:n])[0]
@@ -232,6 +237,7 @@ class SlotPool:
         for s_, d_ in zip(src, dst):
             self.req[d_], self.req[s_] = self.req[s_], None
         self.moves += len(src)
+        self.want_compact = False
         return True
 
     # ---- one SCF iteration of the occupied window --------------------------
@@ -268,9 +274,10 @@ class SlotPool:
         gx = win.g(m.xe[:W])
         m.step(gx, win.wvec, o, nb=W, kbound=kbound)
 
-    def launch(self):
+    def launch(self) -> Launched | None:
         """Enqueue one iteration over the occupied window on the current
-        (pool) stream; returns what ``finish`` needs.  In graph mode the
+        (pool) stream, and the copies of its stop flags and iteration counts
+        to the host; returns what ``finish`` needs.  In graph mode the
         iteration is one replay of a captured graph (captured after an eager
         iteration of the same window)."""
         hi = self.hi
@@ -291,28 +298,35 @@ class SlotPool:
         else:
             live = self.occupied[:W]
             self._iterate(W, kbound=int(self.iters_run - self.adm_it[:W][live].min()) + 1)
+        t = self.iters_run
         self.iters_run += 1
-        # read by finish() once the pool's event completed
-        self.h_flag[:hi].copy_(self.mix.flag[:hi], non_blocking=True)
-        self.h_iter[:hi].copy_(self.mix.ctl[:hi, 3], non_blocking=True)
-        return hi
+        busy = {id(e.h_flag) for e in self.inflight}
+        h_flag, h_iter = next(b for b in self._stage if id(b[0]) not in busy)
+        h_flag[:hi].copy_(self.mix.flag[:hi], non_blocking=True)
+        h_iter[:hi].copy_(self.mix.ctl[:hi, 3], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.dev))
+        return Launched(t, hi, ev, h_flag, h_iter)
 
-    def finish(self, pending, now_fn) -> list[Request]:
-        """Host bookkeeping of a launched iteration; retires stopped slots
-        (converged, or at the iteration limit)."""
-        if pending is None:
+    def finish(self, it: Launched | None, now_fn) -> list[Request]:
+        """Host bookkeeping of a completed iteration: retires the slots it
+        stopped (converged, or at the iteration limit).  Another iteration
+        may be in flight: a molecule that stopped is frozen on the device
+        (the Broyden step skips it), and slots admitted after this iteration
+        was launched are not its molecules."""
+        if it is None:
             return []
-        hi = pending
-        flag = self.h_flag[:hi].numpy().copy()
-        its = self.h_iter[:hi].numpy().astype(np.int64)
-        out = np.nonzero(self.occupied[:hi] & (flag != 0))[0]
+        hi = it.hi
+        flag = it.h_flag[:hi].numpy().copy()
+        its = it.h_iter[:hi].numpy().astype(np.int64)
+        out = np.nonzero(self.occupied[:hi] & (flag != 0) & (self.adm_it[:hi] <= it.t))[0]
         if len(out) == 0:
             return []
         res = self.retire(out, flag[out] == 1, its[out], now_fn)
         # a long-running molecule in a high slot would keep the window wide
         n = int(self.occupied.sum())
         if self.hi - n >= max(4, n // 4):
-            self.compact()
+            self.want_compact = True
         return res
 
     def retire(self, rows: np.ndarray, conv: np.ndarray, iters: np.ndarray, now_fn) -> list[Request]:
@@ -366,6 +380,18 @@ class SlotPool:
 
 
 @dataclass
+class Launched:
+    """An iteration of a pool in flight: its number, window, completion
+    event and the pinned copies of its stop flags and iteration counts."""
+
+    t: int
+    hi: int
+    event: torch.cuda.Event
+    h_flag: torch.Tensor
+    h_iter: torch.Tensor
+
+
+@dataclass
 class Engine:
     """Continuous-batching engine over the slot pools of one parametrization."""
 
@@ -403,17 +429,17 @@ class Engine:
 
     @property
     def busy(self) -> bool:
-        return (any(p.occupied.any() or p.pending is not None or p.retiring for p in self.pools.values())
+        return (any(p.occupied.any() or p.inflight or p.retiring for p in self.pools.values())
                 or any(self.queue.values()))
 
     def _admit_into(self, c: int, p: "SlotPool") -> None:
         """Admit waiting requests of class ``c`` into the free slots of pool
-        ``p`` (no iteration of the pool in flight).  The set-up (integrals,
-        H0, orthogonaliser by the Jacobi solver, solvation) runs on the pool's
-        own stream after the copies of retiring rows, so other pools are not
-        held up."""
+        ``p``.  The set-up (integrals, H0, orthogonaliser by the Jacobi
+        solver, solvation) runs on the pool's own stream after the pool's
+        iterations in flight and the copies of retiring rows, so other pools
+        are not held up."""
         q = self.queue.get(c)
-        if not q or p.pending is not None:
+        if not q:
             return
         m = min(p.nfree, len(q))
         if m:
@@ -429,31 +455,39 @@ class Engine:
                 self._admit_into(c, self.pool(c))
 
     def step(self) -> list[Request]:
-        """One pass of the event loop: admit into new and idle pools, then
-        for every pool whose iteration has completed: finish it (retiring
-        converged molecules), admit waiting requests into the freed slots and
-        launch the next iteration.  Pools advance independently: a pool of
-        small molecules does not wait for a pool of large ones, and a busy
-        pool takes new molecules between two of its iterations.  Pools of
-        larger molecules are served first: their iterations are the longest,
-        so they carry a batch's critical path (in insertion order the
-        config-2 batch ran at 145-172 molecules/s depending on how the pools
-        happened to interleave, largest first at 170-171)."""
+        """One pass of the event loop: admit into new pools, then for every
+        pool: finish its completed iterations (retiring converged molecules),
+        admit waiting requests into the freed slots and keep PIPELINE
+        iterations in flight, so that the pool's stream already holds the
+        next iteration while the host reads the flags of the last one.
+        Compaction waits until the pool has no iteration in flight.  Pools
+        advance independently: a pool of small molecules does not wait for a
+        pool of large ones, and a busy pool takes new molecules between two
+        of its iterations.  Pools of larger molecules are served first: their
+        iterations are the longest, so they carry a batch's critical path (in
+        insertion order the config-2 batch ran at 145-172 molecules/s
+        depending on how the pools happened to interleave, largest first at
+        170-171)."""
         self.admit()
         done = []
         for c in sorted(self.pools, reverse=True):
             p = self.pools[c]
             if p.retiring:
                 done += p.collect(self.now)
-            if p.pending is not None and p.event.query():
+            while p.inflight and p.inflight[0].event.query():
+                it = p.inflight.popleft()
                 with torch.cuda.stream(p.stream):
-                    done += p.finish(p.pending, self.now)
-                p.pending = None
-                self._admit_into(c, p)
-            if p.pending is None and p.occupied.any():
+                    done += p.finish(it, self.now)
+            if p.want_compact and not p.inflight:
                 with torch.cuda.stream(p.stream):
-                    p.pending = p.launch()
-                    p.event.record(p.stream)
+                    p.compact()
+            self._admit_into(c, p)
+            while len(p.inflight) < PIPELINE and p.occupied.any() and not p.want_compact:
+                with torch.cuda.stream(p.stream):
+                    it = p.launch()
+                if it is None:
+                    break
+                p.inflight.append(it)
         return done
 
     def tick(self) -> list[Request]:
@@ -461,8 +495,8 @@ class Engine:
         all)."""
         out = self.step()
         for p in self.pools.values():
-            if p.pending is not None:
-                p.event.synchronize()
+            for it in p.inflight:
+                it.event.synchronize()
         return out + self.step()
 
     def run(self, requests: list[Request], realtime: bool = True, progress=None) -> list[Request]:
