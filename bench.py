@@ -404,7 +404,7 @@ def bench_eigen(args, dev):
         "config": {"workload": f"eigensolver microbenchmark: {B} random symmetric matrices of order {n}, one sweep "
                                "per launch", "kernel_ms": t * 1e3,
                    "fp64_tflops": flops / t / 1e12, "fp64_peak_tflops": FP64_PEAK_TFLOPS},
-        "roofline": {"kernel": "k_bj_check + k_bj_inner + k_bj_product + k_jacobi_blocked (one dxtb_jacobi_eigh call)"
+        "roofline": {"kernel": "k_bj_check + k_bj_inner_full + k_bj_inner_cross + k_bj_product + k_jacobi_blocked (one dxtb_jacobi_eigh call)"
                                if variant == 3 else "k_jacobi_blocked", "bound": "hbm", "achieved": nbytes / t / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": nbytes / t / 1e9 / HBM_PEAK_GBS,
                      "traffic": (JACOBI_TRAFFIC_RATIO_DEFERRED if deferred(n, B) else JACOBI_TRAFFIC_RATIO)[variant]

@@ -417,19 +417,18 @@ __device__ bool inner_sweep(WaveLds& W, double thr, int lane) {
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {  // S <- J^T S J on the 2x2 tile (row pair r + 4u, column pair kk)
-      const double c1 = ca[u], s1 = sa[u];
-      const double lpp = c1 * x[u][0] - s1 * x[u][2], lpq = c1 * x[u][1] - s1 * x[u][3];
-      const double lqp = s1 * x[u][0] + c1 * x[u][2], lqq = s1 * x[u][1] + c1 * x[u][3];
-      W.S[p1[u] * SST + pp] = c2 * lpp - s2 * lpq;
-      W.S[p1[u] * SST + qq] = s2 * lpp + c2 * lpq;
-      W.S[q1[u] * SST + pp] = c2 * lqp - s2 * lqq;
-      W.S[q1[u] * SST + qq] = s2 * lqp + c2 * lqq;
+      tile_rot(ca[u], sa[u], c2, s2, x[u][0], x[u][1], x[u][2], x[u][3]);
+      W.S[p1[u] * SST + pp] = x[u][0];
+      W.S[p1[u] * SST + qq] = x[u][1];
+      W.S[q1[u] * SST + pp] = x[u][2];
+      W.S[q1[u] * SST + qq] = x[u][3];
     }
 #pragma unroll
     for (int v = 0; v < 8; ++v) {  // Q <- Q J: columns p, q of row iq
       const double cq = (v & 1) ? cb[v >> 1] : ca[v >> 1], sq = (v & 1) ? sb[v >> 1] : sa[v >> 1];
-      W.Q[iq * SST + pq[v]] = cq * y[v][0] - sq * y[v][1];
-      W.Q[iq * SST + qv[v]] = sq * y[v][0] + cq * y[v][1];
+      qrow_rot(cq, sq, y[v][0], y[v][1]);
+      W.Q[iq * SST + pq[v]] = y[v][0];
+      W.Q[iq * SST + qv[v]] = y[v][1];
     }
     wave_sync();
   }
@@ -670,6 +669,78 @@ __device__ bool bj_inner_cross(WaveLds& W, const BjMat& M, int k, int s, double 
     }
   }
   wave_sync();
+  return any;
+}
+
+// Coefficient of lane L of this lane's row of 16 for a wave-uniform L in
+// {0, 4, 8, 12} (+1 with odd): row_share needs L at compile time.
+__device__ __forceinline__ double row_share4(double v, int w, bool odd) {
+  if (odd) {
+    switch (w) {
+      case 0: return row_share<1>(v);
+      case 1: return row_share<5>(v);
+      case 2: return row_share<9>(v);
+      default: return row_share<13>(v);
+    }
+  }
+  switch (w) {
+    case 0: return row_share<0>(v);
+    case 1: return row_share<4>(v);
+    case 2: return row_share<8>(v);
+    default: return row_share<12>(v);
+  }
+}
+
+// The full sweep of inner_sweep (all 496 index pairs of the 32x32 subproblem
+// in W, 31 circle-method steps) by four wavefronts: the same lane layout,
+// rotations and update formulas, with wavefront w taking u = w of the
+// one-wavefront form (the 2x2 tile of row pair r + 4w and column pair kk,
+// and the Q columns of pairs r + 4w and r + 1 + 4w of row iq).  Every
+// wavefront computes all 16 rotations of a step from the same LDS values,
+// so they agree on them without an exchange; one workgroup barrier separates
+// those reads from the step's writes (the diagonal tiles hold them), another
+// orders the steps.  Returns whether this wavefront applied any rotation.
+__device__ bool inner_sweep_full4(WaveLds& W, double thr, int tid) {
+  const int w = tid >> 6, lane = tid & 63;
+  const int r = lane >> 4;
+  const int kk = ((lane & 15) + r) & 15;
+  const int iq = lane & 31;
+  bool any = false;
+  for (int s = 0; s < SB - 1; ++s) {
+    int pp, qq;
+    pair32(kk, s, false, pp, qq);
+    double c2, s2;
+    const bool rot = rotation(W.S[pp * SST + pp], W.S[qq * SST + qq], W.S[pp * SST + qq], thr, c2, s2);
+    const bool step_any = __any(rot ? 1 : 0);  // the same in every wavefront (same values, same arithmetic)
+    // every wavefront has read the diagonal entries of the step's pairs before
+    // any wavefront overwrites them (the diagonal 2x2 tiles are updated below)
+    __syncthreads();
+    if (step_any) {
+      any = true;
+      const double ca = row_share4(c2, w, false), sa = row_share4(s2, w, false);
+      const double cb = row_share4(c2, w, true), sb = row_share4(s2, w, true);
+      int p1, q1, pa, qa, pb, qb;
+      pair32(r + 4 * w, s, false, p1, q1);
+      pair32(r + 4 * w, s, false, pa, qa);
+      pair32((r + 1 + 4 * w) & 15, s, false, pb, qb);
+      double x0 = W.S[p1 * SST + pp], x1 = W.S[p1 * SST + qq];
+      double x2 = W.S[q1 * SST + pp], x3 = W.S[q1 * SST + qq];
+      double ya0 = W.Q[iq * SST + pa], ya1 = W.Q[iq * SST + qa];
+      double yb0 = W.Q[iq * SST + pb], yb1 = W.Q[iq * SST + qb];
+      tile_rot(ca, sa, c2, s2, x0, x1, x2, x3);
+      W.S[p1 * SST + pp] = x0;
+      W.S[p1 * SST + qq] = x1;
+      W.S[q1 * SST + pp] = x2;
+      W.S[q1 * SST + qq] = x3;
+      qrow_rot(ca, sa, ya0, ya1);
+      qrow_rot(cb, sb, yb0, yb1);
+      W.Q[iq * SST + pa] = ya0;
+      W.Q[iq * SST + qa] = ya1;
+      W.Q[iq * SST + pb] = yb0;
+      W.Q[iq * SST + qb] = yb1;
+    }
+    __syncthreads();
+  }
   return any;
 }
 
@@ -1097,16 +1168,42 @@ __global__ __launch_bounds__(JBB) void k_bj_check(int nmax, const JArgs* __restr
   }
 }
 
-__global__ __launch_bounds__(64) void k_bj_inner(int nmax, const JArgs* __restrict__ ja, double* __restrict__ work,
-                                                const double* __restrict__ sst, int* __restrict__ rot, int s) {
+// Inner phase of the split form at outer step 0: one four-wavefront
+// workgroup per (block pair, matrix) stages the 32x32 subproblem, runs the
+// full sweep (inner_sweep_full4) and writes back the diagonal tile and the
+// rotation, as bj_inner_pair does with one wavefront.
+__global__ __launch_bounds__(CW * 64) void k_bj_inner_full(int nmax, const JArgs* __restrict__ ja,
+                                                          double* __restrict__ work, const double* __restrict__ sst,
+                                                          int* __restrict__ rot) {
   __shared__ WaveLds W;
-  const int b = blockIdx.y, k = blockIdx.x;
+  const int b = blockIdx.y, k = blockIdx.x, tid = threadIdx.x;
   const JArgsC jc = jconst(ja);
   if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
   const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, rot, jc->defer);
-  if (k >= M.npairs || s >= M.nb - 1) return;
-  const bool any = bj_inner_pair(W, M, k, s, sst[4 * b + 1], threadIdx.x);
-  if (threadIdx.x == 0) fstep(M, s)[k] = any ? 1 : 0;
+  if (k >= M.npairs || M.nb < 2) return;
+  int bi, bj;
+  block_pair(k, 0, M.nb, bi, bj);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = tid + CW * 64 * u;
+    const int gr = gidx(e >> 5, bi, bj), gc = gidx(e & 31, bi, bj);
+    const double x = M.A[(size_t)min(gr, M.nlast) * M.nmax + min(gc, M.nlast)];
+    W.S[(e >> 5) * SST + (e & 31)] = x * mask01(gr < M.n && gc < M.n);
+    W.Q[(e >> 5) * SST + (e & 31)] = ((e >> 5) == (e & 31)) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const bool any = __syncthreads_or(inner_sweep_full4(W, sst[4 * b + 1], tid) ? 1 : 0) != 0;
+  if (any) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + CW * 64 * u;
+      const int r = e >> 5, c = e & 31;
+      const int gr = gidx(r, bi, bj), gc = gidx(c, bi, bj);
+      if (gr < M.n && gc < M.n) M.A[(size_t)gr * M.nmax + gc] = W.S[r * SST + c];
+      qstep(M, 0)[(size_t)k * SB * SB + e] = W.Q[r * SST + c];
+    }
+  }
+  if (tid == 0) fstep(M, 0)[k] = any ? 1 : 0;
 }
 
 // Inner phase of the split form at outer steps s > 0: one CW-wavefront
@@ -1215,8 +1312,8 @@ hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax,
       hipLaunchKernelGGL(k_bj_check, dim3((unsigned)nbcap), dim3(JBB), 0, st, nmax, w.ja, w.rot_q, tol, w.sst, sw);
       for (int s = 0; s < nb - 1; ++s) {
         if (s == 0)  // the full sweep of every subproblem (from LDS), then the cross sweeps
-          hipLaunchKernelGGL(k_bj_inner, dim3((unsigned)npairs, (unsigned)nbcap), dim3(64), 0, st, nmax, w.ja,
-                             w.rot_q, w.sst, w.rot, s);
+          hipLaunchKernelGGL(k_bj_inner_full, dim3((unsigned)npairs, (unsigned)nbcap), dim3(CW * 64), 0, st, nmax,
+                             w.ja, w.rot_q, w.sst, w.rot);
         else
           hipLaunchKernelGGL(k_bj_inner_cross, dim3((unsigned)npairs, (unsigned)nbcap), dim3(CW * 64), 0, st, nmax,
                              w.ja, w.rot_q, w.sst, w.rot, s);
