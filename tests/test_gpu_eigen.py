@@ -148,6 +148,40 @@ def test_split_bitwise_matches_blocked(cuda, B, n):
         assert torch.equal(e2, e3) and torch.equal(V2, V3)
 
 
+def test_split_concurrent_streams_bitwise(cuda):
+    """Split-form solves of two batches on two streams at once, beside a
+    large product on a third, give bit for bit the results of the same
+    solves run alone: the kernels' LDS exchanges do not depend on how their
+    wavefronts are scheduled (a missing barrier shows up here as different
+    rotations under contention)."""
+    print("case jacobi split concurrent streams", flush=True)
+    g = torch.Generator().manual_seed(17)
+    mats = []
+    for B, n in ((48, 256), (24, 320)):
+        A = torch.zeros(B, n, n, dtype=torch.float64)
+        for b in range(B):
+            d = torch.sort(torch.randn(n, generator=g, dtype=torch.float64)).values
+            M = 1e-2 * torch.randn(n, n, generator=g, dtype=torch.float64)
+            A[b] = torch.diag(d) + 0.5 * (M + M.T)
+        mats.append(A.to(cuda))
+    solo = [_jacobi(A, [A.shape[1]] * A.shape[0], 3, tol=1e-12, split=12) for A in mats]
+    big = torch.randn(4096, 4096, dtype=torch.float64, device=cuda)
+    streams = [torch.cuda.Stream(cuda) for _ in range(3)]
+    torch.cuda.synchronize(cuda)
+    for _ in range(3):
+        outs = []
+        with torch.cuda.stream(streams[2]):
+            c = big @ big
+        for A, st in zip(mats, streams[:2]):
+            with torch.cuda.stream(st):
+                outs.append(_jacobi(A, [A.shape[1]] * A.shape[0], 3, tol=1e-12, split=12))
+        torch.cuda.synchronize(cuda)
+        del c
+        for (e0, V0, s0), (e1, V1, s1) in zip(solo, outs):
+            assert torch.equal(s0, s1)
+            assert torch.equal(e0, e1) and torch.equal(V0, V1)
+
+
 def test_split_few_sweeps(cuda):
     """max_sweeps below the split sweeps: the split form stops there too."""
     n = 200
