@@ -1146,19 +1146,61 @@ __global__ __launch_bounds__(JBB) void k_bj_init(int nmax, const JArgs* __restri
   if (threadIdx.x < 4) sst[4 * b + threadIdx.x] = 0.0;
 }
 
-__global__ __launch_bounds__(JBB) void k_bj_check(int nmax, const JArgs* __restrict__ ja, double* __restrict__ work,
-                                                 double tol, double* __restrict__ sst, int sweep) {
-  __shared__ double red[NWB];
-  const int b = blockIdx.x;
+// Convergence test of the split form, in two launches: k_bj_norms reduces
+// chunks of about 16384 elements (whole rows) of every matrix over many
+// workgroups, k_bj_check combines a matrix's chunks and decides whether it
+// stops.  (One workgroup reading a whole matrix took 2.2 ms at order 4002.)
+__host__ __device__ constexpr int norm_rows(int nmax) { return nmax >= 16384 ? 1 : 16384 / nmax; }
+__host__ __device__ constexpr int norm_chunks(int nmax) { return (nmax + norm_rows(nmax) - 1) / norm_rows(nmax); }
+constexpr int NRB = 256;  // threads per k_bj_norms workgroup
+
+__global__ __launch_bounds__(NRB) void k_bj_norms(int nmax, const JArgs* __restrict__ ja,
+                                                 const double* __restrict__ sst, double* __restrict__ nrm) {
+  __shared__ double red[NRB / 64];
+  const int b = blockIdx.x, c = blockIdx.y;
+  const JArgsC jc = jconst(ja);
+  if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
+  const int n = min(max(jc->nsize[b], 0), nmax);
+  const double* __restrict__ A = jc->A + (size_t)b * nmax * nmax;
+  const int r0 = c * norm_rows(nmax), r1 = min(n, r0 + norm_rows(nmax));
+  double off = 0.0, amax = 0.0;
+  const int ncol = n, nel = max(r1 - r0, 0) * ncol;
+  for (int e = threadIdx.x; e < nel; e += NRB) {
+    const int i = r0 + e / ncol, j = e - (e / ncol) * ncol;
+    const double x = fabs(A[(size_t)i * nmax + j]);
+    off = fmax(off, i == j ? 0.0 : x);
+    amax = fmax(amax, i == j ? x : 0.0);
+  }
+  off = block_max<NRB / 64>(off, red);
+  amax = block_max<NRB / 64>(amax, red);
+  if (threadIdx.x == 0) {
+    double* o = nrm + ((size_t)b * norm_chunks(nmax) + c) * 2;
+    o[0] = off;
+    o[1] = amax;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_bj_check(int nmax, const JArgs* __restrict__ ja, const double* __restrict__ nrm,
+                                                double tol, double* __restrict__ sst, int sweep) {
+  const int b = blockIdx.x, lane = threadIdx.x;
   const JArgsC jc = jconst(ja);
   if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
   const double* __restrict__ tol_b = jc->tol_b;
-  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, nullptr, 0);
-  double off, amax;
-  bj_norms(M, red, off, amax);
+  const int nch = norm_chunks(nmax);
+  double off = 0.0, amax = 0.0;
+  for (int c = lane; c < nch; c += 64) {
+    off = fmax(off, nrm[((size_t)b * nch + c) * 2]);
+    amax = fmax(amax, nrm[((size_t)b * nch + c) * 2 + 1]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    off = fmax(off, __shfl_xor(off, o, 64));
+    amax = fmax(amax, __shfl_xor(amax, o, 64));
+  }
+  const int n = min(max(jc->nsize[b], 0), nmax);
   const double thr = fmax(1e-300, (tol_b ? tol_b[b] : tol) * fmax(amax, 1.0));
-  if (threadIdx.x == 0) {
-    if (M.n == 0 || stop_sweeps(off, sst[4 * b + 0], thr, sweep)) {
+  if (lane == 0) {
+    if (n == 0 || stop_sweeps(off, sst[4 * b + 0], thr, sweep)) {
       sst[4 * b + 2] = 1.0;
     } else {
       sst[4 * b + 0] = off;
@@ -1287,7 +1329,13 @@ struct Layout {
   double* rot_q;  // rotations of the current outer step
   double* sst;    // split state
   int* rot;       // rotation flags
+  double* nrm;    // per matrix and row chunk: largest off-diagonal and diagonal magnitude
 };
+
+size_t flag_bytes(int nbcap, int nmax) {
+  return ((size_t)nbcap * log_steps(nmax) * KMAX * sizeof(int) + 255) / 256 * 256;
+}
+size_t norm_bytes(int nbcap, int nmax) { return (size_t)nbcap * norm_chunks(nmax) * 2 * sizeof(double); }
 
 Layout layout(double* work, int nbcap, int nmax) {
   Layout w;
@@ -1296,6 +1344,7 @@ Layout layout(double* work, int nbcap, int nmax) {
   w.rot_q = reinterpret_cast<double*>(p + JARGS_BYTES);
   w.sst = w.rot_q + rot_bytes(nbcap, nmax) / sizeof(double);
   w.rot = reinterpret_cast<int*>(w.sst + 4 * (size_t)nbcap);
+  w.nrm = reinterpret_cast<double*>(reinterpret_cast<char*>(w.rot) + flag_bytes(nbcap, nmax));
   return w;
 }
 
@@ -1309,7 +1358,9 @@ hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax,
   if (nsw > 0) {
     hipLaunchKernelGGL(k_bj_init, dim3((unsigned)nbcap), dim3(JBB), 0, st, nmax, w.ja, w.sst);
     for (int sw = 0; sw < nsw; ++sw) {
-      hipLaunchKernelGGL(k_bj_check, dim3((unsigned)nbcap), dim3(JBB), 0, st, nmax, w.ja, w.rot_q, tol, w.sst, sw);
+      hipLaunchKernelGGL(k_bj_norms, dim3((unsigned)nbcap, (unsigned)norm_chunks(nmax)), dim3(NRB), 0, st, nmax,
+                         w.ja, w.sst, w.nrm);
+      hipLaunchKernelGGL(k_bj_check, dim3((unsigned)nbcap), dim3(64), 0, st, nmax, w.ja, w.nrm, tol, w.sst, sw);
       for (int s = 0; s < nb - 1; ++s) {
         if (s == 0)  // the full sweep of every subproblem (from LDS), then the cross sweeps
           hipLaunchKernelGGL(k_bj_inner_full, dim3((unsigned)npairs, (unsigned)nbcap), dim3(CW * 64), 0, st, nmax,
@@ -1425,8 +1476,8 @@ extern "C" int dxtb_jacobi_max_n(void) { return JMAX; }
 extern "C" size_t dxtb_jacobi_work_size(int nbatch, int nmax) {
   if (nbatch <= 0 || nmax <= 0) return 0;
   const int nbcap = batch_capacity(nbatch);
-  return JARGS_BYTES + rot_bytes(nbcap, nmax) + (size_t)nbcap * 4 * sizeof(double) +
-         (size_t)nbcap * log_steps(nmax) * KMAX * sizeof(int);
+  return JARGS_BYTES + rot_bytes(nbcap, nmax) + (size_t)nbcap * 4 * sizeof(double) + flag_bytes(nbcap, nmax) +
+         norm_bytes(nbcap, nmax);
 }
 
 extern "C" int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals,

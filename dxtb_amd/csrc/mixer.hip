@@ -26,10 +26,10 @@
 // workgroup per molecule was latency-bound):
 //   k_bro_dots   one wavefront per history entry k: <c_k, w ddx>,
 //                <d_k, w ddy>, <d_k, w ynew>;
-//   k_bro_comb   per chunk of 64 elements and group of entries: the partial
-//                combinations sum_k <c_k, w ddx> d_k, sum_k <d_k, w ddy> c_k,
-//                sum_k <d_k, w ynew> c_k (one slice of entries per wavefront,
-//                added in LDS);
+//   k_bro_comb   per chunk of 64 elements and group of CS KPT consecutive
+//                entries: the partial combinations sum_k <c_k, w ddx> d_k,
+//                sum_k <d_k, w ddy> c_k, sum_k <d_k, w ynew> c_k (one slice of
+//                entries per wavefront, added in LDS);
 //   k_bro_update one workgroup per molecule: G^T ddx, G ddy, G ynew from the
 //                partials, the rank-one update, the next point and the stop
 //                test.
@@ -136,8 +136,12 @@ __global__ __launch_bounds__(CE * CS) void k_bro_comb(MixArgs a) {
   const double* __restrict__ coef = a.work + (size_t)b * work_stride(L, H);
   double v = 0.0, gdy = 0.0, gy = 0.0;
   if (i < L) {
-    const int stride = a.kg * CS;
-    for (int k = kg * CS + sl; k < K; k += stride) {
+    // group kg sums the entries [kg CS KPT, (kg + 1) CS KPT) (the last group
+    // also any beyond), slice sl every CS-th of them: a fixed summation order
+    // for every entry whatever the number of groups, so a result does not
+    // depend on the caller's history bound (extra groups add zeros)
+    const int k1 = (kg == a.kg - 1) ? K : min(K, (kg + 1) * CS * KPT);
+    for (int k = kg * CS * KPT + sl; k < k1; k += CS) {
       const double c = a.hc[((size_t)b * H + k) * L + i];
       const double d = a.hd[((size_t)b * H + k) * L + i];
       v += coef[k] * d;

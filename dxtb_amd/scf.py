@@ -109,7 +109,7 @@ FULL_MODES = ("full", "full_tracking", "full-tracking")
 
 class _SolveHistory:
     """Per-state history of the Jacobi solves: the sweeps the next solve is
-    expected to need, and when to re-orthogonalise the accumulated rotation.
+    expected to need.
 
     dxtb_jacobi_eigh runs that many sweeps in its split form (grid-wide
     launches over many workgroups per matrix, for a few large matrices) and
@@ -139,7 +139,6 @@ class _SolveHistory:
         self.fresh_sweeps = self.FRESH + self.margin - 1 + (6 if n > 1536 else 0)
         self.recent: dict[int, list[int]] = {}  # size class -> last counts
         self.bump = False
-        self.calls = 0
         self.sw = None
         # counts copied back without blocking: (event, pinned buffer, rows,
         # size class) in launch order; a solve's count is used once its copy
@@ -170,12 +169,6 @@ class _SolveHistory:
 
     def fresh(self) -> None:
         self.bump = True
-
-    def reorthogonalise(self, every: int = 4) -> bool:
-        """Whether this solve re-orthogonalises the accumulated rotation (every
-        ``every``-th solve; always inside a graph capture, which is replayed)."""
-        self.calls += 1
-        return torch.cuda.is_current_stream_capturing() or self.calls % every == 0
 
     def counts(self, nb: int, dev) -> torch.Tensor:
         if self.sw is None or self.sw.numel() < nb:
@@ -345,7 +338,7 @@ class GeometryState:
         # orthogonal (None: identity), U <- U V^T after every solve.  Each V is
         # orthogonal to ~1e-13 only, and the products drift linearly (3e-11
         # after 170 iterations of a 628-function molecule: 2e-8 surplus
-        # electrons, 1e-8 Eh), so U is re-orthogonalised every few solves
+        # electrons, 1e-8 Eh), so U is re-orthogonalised at every solve
         self.U = None
         # the previous iteration's orbitals C = X U (None: X), kept so that
         # the next solve starts from them without recomputing X U
@@ -578,15 +571,15 @@ class GeometryState:
                 ev[1].record(torch.cuda.current_stream(self.dev))
                 prof.append((ev[0], ev[1], n, np.array(self.bb.nao), sw.clone(), variant, split))
             # V holds the eigenvectors as rows: C V^T and U V^T
-            C = gemm(Cb, V, transb=True, nsize=nsz, pad=1.0)
             Un = V.transpose(1, 2).contiguous() if self.U is None else gemm(self.U, V, transb=True, nsize=nsz,
                                                                              pad=1.0)
-            if self._jhist.reorthogonalise():
-                # one Newton-Schulz step, U (3 - U^T U) / 2: U^T U = 1 + E becomes 1 + O(E^2);
-                # the orbitals are then rebuilt from the re-orthogonalised U
-                G = gemm(Un, Un, transa=True, nsize=nsz, sym=True, pad=1.0)
-                Un = gemm(Un, G, alpha=-0.5, beta=1.5, cin=Un, nsize=nsz, pad=1.0)
-                C = gemm(self.X, Un, nsize=nsz, pad=1.0)
+            # one Newton-Schulz step every solve, U (3 - U^T U) / 2: U^T U = 1 + E
+            # becomes 1 + O(E^2); the orbitals are built from the re-orthogonalised
+            # U.  (Every solve, not every k-th: a molecule's arithmetic then does
+            # not depend on when it joined a batch.)
+            G = gemm(Un, Un, transa=True, nsize=nsz, sym=True, pad=1.0)
+            Un = gemm(Un, G, alpha=-0.5, beta=1.5, cin=Un, nsize=nsz, pad=1.0)
+            C = gemm(self.X, Un, nsize=nsz, pad=1.0)
             if self.U is None:
                 self.U = Un
                 self.Corb = C

@@ -52,6 +52,9 @@ def test_alone_vs_mix(cuda):
         print(f"case {r.rid}: iterations mix {m.iterations} alone {alone.iterations} plain {ref.iterations}, "
               f"dE {m.energy - alone.energy:.2e} {m.energy - ref.energy:.2e}", flush=True)
         assert m.converged and alone.converged and ref.converged
+        # a molecule's SCF arithmetic does not depend on its batch mates or on
+        # when it joined its pool: the same iterations alone and in the mix
+        assert m.iterations == alone.iterations
         for other in (alone, ref):
             assert abs(m.energy - other.energy) < 1e-9
             np.testing.assert_allclose(m.forces, other.forces, atol=1e-7, rtol=0)
