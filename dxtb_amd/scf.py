@@ -180,11 +180,8 @@ class _SolveHistory:
             return
         if len(self.pending) >= self.PENDING:
             return  # the host is far ahead: this count is not kept
-        buf = next((b for b in self.free if b.numel() >= nb), None)
-        if buf is None:
-            buf = torch.zeros(max(nb, 8), dtype=torch.int32, pin_memory=True)
-        else:
-            self.free.remove(buf)
+        i = next((i for i, b in enumerate(self.free) if b.numel() >= nb), None)
+        buf = torch.zeros(max(nb, 8), dtype=torch.int32, pin_memory=True) if i is None else self.free.pop(i)
         buf[:nb].copy_(self.sw[:nb], non_blocking=True)
         ev = torch.cuda.Event()
         ev.record(torch.cuda.current_stream(dev))

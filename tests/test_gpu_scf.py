@@ -238,6 +238,25 @@ def test_forces_central_differences_config2(cuda):
     print(f"case config-2 forces vs central differences: max |dF| {worst:.1e}", flush=True)
 
 
+def test_sweep_history_queue(cuda):
+    """The split-sweep predictor keeps every sweep-count readback until its
+    copy has completed, over windows of changing sizes (pinned buffers of
+    different lengths are reused): each prediction is the largest count of
+    the size class's recent solves plus the margin."""
+    h = hscf._SolveHistory(256)
+    assert h.predict(12) == h.fresh_sweeps
+    for k, nb in enumerate((12, 8, 12, 3, 20, 8, 30, 12, 5)):
+        sw = h.counts(nb, cuda)
+        sw.fill_(k + 1)
+        h.record(nb, cuda)
+        torch.cuda.synchronize(cuda)
+        got = h.predict(nb)
+        hist = [j + 1 for j, m in enumerate((12, 8, 12, 3, 20, 8, 30, 12, 5)[: k + 1])
+                if h.size_class(m) == h.size_class(nb)][-h.HISTORY:]
+        assert got == max(hist) + h.margin, (nb, got, hist)
+    assert not h.pending
+
+
 def test_water_cluster_config4(cuda):
     """Config 4 at full size: 667 rigid waters (2001 atoms, 4002 basis
     functions; the hand-written Jacobi solver in its split form for the
