@@ -203,13 +203,16 @@ def jacobi_flops_per_sweep(n: int, variant: int) -> float:
 # measured HBM traffic of one dxtb_jacobi_eigh call / its algorithmic bytes,
 # by form: rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch (summed over the
 # call's kernels), scaled by the same counters on a copy of known size (eigen
-# workload, 256 x order 240, one sweep; FETCH_SIZE x 2, WRITE_SIZE x 1).
-# 2: one workgroup per matrix; 3 (and 0, which picks 3 or 2): the split form;
-# both with the deferred eigenvector update there (256 matrices).  The scalar
-# form (1, A in LDS) is not measured and counted at its algorithmic bytes.
-JACOBI_TRAFFIC_RATIO = {0: 1.103, 1: 1.0, 2: 1.154, 3: 1.103}
-# the same with the deferred eigenvector update, against its algorithmic bytes
-JACOBI_TRAFFIC_RATIO_DEFERRED = {0: 1.170, 1: 1.0, 2: 1.542, 3: 1.170}
+# workload, order 240, one sweep; FETCH_SIZE x 2, WRITE_SIZE x 1).
+# 2: one workgroup per matrix; 3 (and 0, which picks 3 or 2): the split form.
+# Per-step eigenvector update: 16 matrices (split form; the product reads only
+# the upper tiles of A, hence below 1); the one-workgroup value is from 256
+# matrices before the deferred update existed.  The scalar form (1, A in LDS)
+# is not measured and counted at its algorithmic bytes.
+JACOBI_TRAFFIC_RATIO = {0: 0.968, 1: 1.0, 2: 1.154, 3: 0.968}
+# the same with the deferred eigenvector update (256 matrices), against its
+# algorithmic bytes
+JACOBI_TRAFFIC_RATIO_DEFERRED = {0: 1.171, 1: 1.0, 2: 1.542, 3: 1.171}
 # the library's choice of the deferred eigenvector update (eigen.hip:
 # defer_call): order <= 512 and a batch capacity above 16
 DEFER_MAX, DEFER_MIN_BATCH = 512, 16
