@@ -51,16 +51,25 @@ def test_jacobi_arguments_without_gpu():
     # capacity the 32x32 rotations of the order padded to a multiple of 32 for
     # every step of a sweep (order 33: 2 * 2 - 1 = 3 steps of 64 rows; the
     # whole sweep's are kept up to order 512), 4 doubles of split-sweep state,
-    # and per step 128 int rotation flags (block pairs of the largest order, 4096)
+    # per step 128 int rotation flags (block pairs of the largest order, 4096;
+    # the flags of the batch rounded up to 256 bytes), and two doubles per
+    # chunk of rows of the convergence test (about 16384 elements per chunk)
     assert lib.dxtb_jacobi_max_n() == 4096
-    per = 3 * 64 * 32 * 8 + 4 * 8 + 3 * 128 * 4
-    assert lib.dxtb_jacobi_work_size(2, 33) == 256 + 2 * per
+
+    def size(cap, n, steps, npad):
+        flags = -(-cap * steps * 128 * 4 // 256) * 256
+        chunks = -(-n // max(1, 16384 // n))
+        return 256 + cap * steps * npad * 32 * 8 + cap * 4 * 8 + flags + cap * chunks * 2 * 8
+
+    assert lib.dxtb_jacobi_work_size(2, 33) == size(2, 33, 3, 64)
     # capacities: powers of two up to 8, multiples of 8 up to 64, then of 32
     for nb, cap in ((1, 1), (3, 4), (5, 8), (9, 16), (63, 64), (65, 96), (256, 256), (257, 288)):
-        assert lib.dxtb_jacobi_work_size(nb, 33) == 256 + cap * per, nb
+        assert lib.dxtb_jacobi_work_size(nb, 33) == size(cap, 33, 3, 64), nb
     assert lib.dxtb_jacobi_work_size(0, 33) == 0
-    # above order 512 the rotations of one step only: order 600, 608 rows
-    assert lib.dxtb_jacobi_work_size(1, 600) == 256 + 608 * 32 * 8 + 4 * 8 + 128 * 4
+    # above order 512 the rotations of one step only: order 600, 608 rows,
+    # 23 chunks of 27 rows
+    assert lib.dxtb_jacobi_work_size(1, 600) == size(1, 600, 1, 608)
+    assert size(1, 600, 1, 608) == 256 + 608 * 32 * 8 + 4 * 8 + 512 + 23 * 16
     assert lib.dxtb_jacobi_max_n() >= 128
     args = (None,) * 5
     assert lib.dxtb_jacobi_eigh(0, 10, *args, 0, 10, 0, 1e-14, None, None, None) == _lib.DXTB_EINVAL
