@@ -311,7 +311,7 @@ def jacobi_roofline(records, base) -> dict:
     achieved = canon / total_s / 1e12
     hbm = nbytes / total_s / 1e9
     return {
-        "kernel": "dxtb_jacobi_eigh (split form k_bj_* + k_jacobi_blocked; k_jacobi_scalar up to order 64)",
+        "kernel": "dxtb_jacobi_eigh (split form k_bj_* + k_jacobi_blocked above order 96; k_jacobi_blocked, one workgroup per matrix, up to 96)",
         "bound": "mfma",
         "achieved": achieved,
         "peak": FP64_PEAK_TFLOPS,
@@ -775,7 +775,7 @@ def main():
     ap.add_argument("--ao-order", default="l", choices=["l", "reference"],
                     help="integrals workload: AO layout of the matrices")
     ap.add_argument("--jacobi-variant", type=int, default=3, choices=[0, 2, 3],
-                    help="eigensolver form above order 64: 3 split (default), 2 one workgroup per matrix, 0 auto")
+                    help="eigensolver form above order 96: 3 split (default), 2 one workgroup per matrix, 0 auto")
     args = ap.parse_args()
     if args.trace_summary:
         trace_summary(args.trace_summary, args.trace_steps)

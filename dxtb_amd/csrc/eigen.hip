@@ -49,7 +49,6 @@ constexpr int JBB = 512;               // threads per workgroup of the blocked v
 constexpr int NWB = JBB / 64;          // its wavefronts (one block pair or product item each)
 constexpr int JMAX_LDS = 128;          // largest matrix of the scalar variant
 constexpr int JMAX = 4096;             // largest matrix overall
-constexpr int JSCALAR_AUTO = 64;       // auto: scalar variant up to this order
 constexpr int BS = 16;                 // block size of the blocked variant
 constexpr int SB = 2 * BS;             // subproblem order
 constexpr int SST = SB + 1;            // LDS stride of a subproblem
@@ -1511,7 +1510,7 @@ int jacobi_impl(int nbatch, int nmax, const int* nsize, double* A, double* VT, d
   if (nbatch <= 0 || nmax <= 0 || nmax > JMAX || !nsize || !A || !VT || !evals || max_sweeps < 0 || variant < 0 ||
       variant > 3 || split_sweeps < 0)
     return DXTB_EINVAL;
-  const bool scalar = variant == 1 || (variant == 0 && nmax <= JSCALAR_AUTO);
+  const bool scalar = variant == 1;
   if (scalar && nmax > JMAX_LDS) return DXTB_EINVAL;
   if (!scalar && !work) return DXTB_EINVAL;
   int dev = 0;

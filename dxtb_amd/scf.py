@@ -53,9 +53,13 @@ JACOBI_TOL_LOOSE = 1.0e-6
 
 def jacobi_tolerance(res: torch.Tensor) -> torch.Tensor:
     return (JACOBI_RES_FACTOR * res).clamp(JACOBI_TOL, JACOBI_TOL_LOOSE).contiguous()
-# orders up to this use the scalar LDS variant of the Jacobi solver, larger
-# ones the blocked matrix-core variant
-JACOBI_SCALAR_MAX = 64
+# orders up to this use the blocked Jacobi solver with one workgroup per
+# matrix (variant 2), larger ones SCFOptions.jacobi_variant.  Measured on
+# warm-start-like batches (4 sweeps): order 48, 0.31-0.36 ms against 0.63-0.67
+# with the scalar LDS variant and 0.33-0.44 split; order 96, 0.53-0.65 ms
+# against 3.7 scalar and 0.53-0.75 split.  The blocked forms agree bit for bit,
+# so the choice does not change a result.
+JACOBI_ONE_WG_MAX = 96
 # when a list, solve() appends (start event, end event, order, basis sizes,
 # sweeps tensor, variant) for every Jacobi launch (kernel timing on the
 # launch stream; used by the benchmark)
@@ -82,9 +86,10 @@ class SCFOptions:
     # Broyden iterations than they save in sweeps (750 instead of 1011 of 1024
     # converged within 250 iterations)
     adaptive_eigensolver: bool = False
-    # dxtb_jacobi_eigh form above order 64: 3 the split form (sweeps spread
-    # over many workgroups per matrix), 2 one workgroup per matrix, 0 the
-    # library's choice for a lone batch (split below half the basis size).
+    # dxtb_jacobi_eigh form above order JACOBI_ONE_WG_MAX: 3 the split form
+    # (sweeps spread over many workgroups per matrix), 2 one workgroup per
+    # matrix, 0 the library's choice for a lone batch (split below half the
+    # basis size), 1 the scalar LDS form (order <= 128).
     # The split form's small workgroups share the device with the other slot
     # pools' kernels: config-2 batch 132.7 molecules/s against 117.6 with 2
     # and 116.7 with 0
@@ -209,7 +214,7 @@ def _orthogonaliser(spad: torch.Tensor, nsize: torch.Tensor, variant: int = 3) -
     VT = torch.empty_like(A)
     ev = torch.empty((B, n), dtype=torch.float64, device=dev)
     work = torch.empty(max(lib.dxtb_jacobi_work_size(B, n) // 8, 1), dtype=torch.float64, device=dev)
-    var = 1 if n <= JACOBI_SCALAR_MAX else variant
+    var = 2 if n <= JACOBI_ONE_WG_MAX else variant
     split = S_SPLIT_SWEEPS + (6 if n > 1536 else 0)
     max_sweeps = 60 if n <= 1536 else split  # one-workgroup sweeps take seconds above order 1536
     rc = lib.dxtb_jacobi_eigh(B, n, _lib.ptr(nsize), _lib.ptr(A), _lib.ptr(VT), _lib.ptr(ev), _lib.ptr(work), var,
@@ -544,8 +549,8 @@ class GeometryState:
             nbytes = self.lib.dxtb_jacobi_work_size(self.bb.nbatch, n)
             if self._jwork is None or self._jwork.numel() * 8 < nbytes:
                 self._jwork = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=self.dev)
-            # scalar form up to order 64, else a blocked form (SCFOptions.jacobi_variant)
-            variant = 1 if n <= JACOBI_SCALAR_MAX else self.opts.jacobi_variant
+            # one workgroup per matrix up to JACOBI_ONE_WG_MAX, else SCFOptions.jacobi_variant
+            variant = 2 if n <= JACOBI_ONE_WG_MAX else self.opts.jacobi_variant
             nb = self.bb.nbatch
             # a captured graph replays over many iterations and admissions: FRESH
             fresh = self.U is None or torch.cuda.is_current_stream_capturing()

@@ -306,9 +306,11 @@ size_t dxtb_jacobi_work_size(int nbatch, int nmax);
  * memory), one workgroup per matrix; variant 3: the blocked Jacobi whose
  * first split_sweeps sweeps run as grid-wide launches over many workgroups
  * per matrix (for a few large matrices; converged matrices skip the rest),
- * finished by variant 2; variant 0 picks: 1 up to nmax 64, 3 for
- * nmax >= 128 and nbatch < nmax / 2 (where it is faster on a lone batch),
- * else 2.  split_sweeps (>= 0) is best set to
+ * finished by variant 2; variant 0 picks 3 for nmax >= 128 and
+ * nbatch < nmax / 2 (where it is faster on a lone batch), else 2 (the
+ * scalar variant, slower than the blocked ones at every order measured, only
+ * when asked for).  Variants 2 and 3 give bit-identical results for any
+ * split_sweeps.  split_sweeps (>= 0) is best set to
  * the sweeps the caller expects (e.g. of the previous SCF iteration).
  * A (nbatch, nmax, nmax): only the leading nsize[b] block of every matrix is
  * used; the blocked variant overwrites it.  VT (nbatch, nmax, nmax) receives
