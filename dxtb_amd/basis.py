@@ -64,6 +64,9 @@ class ShellTable:
     def __init__(self, gfn: str) -> None:
         par = param.get(gfn)
         self.gfn = gfn
+        # the second s shell of H is orthogonalised to the first in GFN1
+        # (basis/bas.py:147-163: meta.name of the GFN0/GFN1 family)
+        gfn1 = par.kind == "gfn1"
         self.first = np.full(param.MAXZ + 1, -1, dtype=np.int64)  # first unique shell of Z
         self.count = np.zeros(param.MAXZ + 1, dtype=np.int64)
         ls, nps, al, co, zs = [], [], [], [], []
@@ -78,7 +81,7 @@ class ShellTable:
             prev = None
             for i, (n, l) in enumerate(par.shells(z)):
                 a, c = slater_to_gauss(int(e["ngauss"][i]), n, l, float(e["slater"][i]))
-                if gfn == "gfn1" and not val[i]:
+                if gfn1 and not val[i]:
                     a, c = orthogonalize(prev[0], prev[1], a, c)
                 prev = (a, c)
                 # exact unit self overlap (impls/pairs.py:181-204)

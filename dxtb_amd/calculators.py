@@ -1,6 +1,8 @@
 """
-Calculator facade of the HIP path: ``GFN1Calculator`` / ``GFN2Calculator``
-with ``get_energy`` and ``get_forces``.
+Calculator facade of the HIP path: ``Calculator(numbers, par)`` for any
+GFN1/GFN2-type parametrization, ``GFN1Calculator`` / ``GFN2Calculator`` with
+the shipped ones, with ``get_energy``, ``get_forces`` and the single-point
+properties.
 
 Mirrors the reference calculator API (``src/dxtb/_src/calculators/gfn1.py:38-70``,
 ``gfn2.py:38-69``, ``types/abc.py:88-170``, ``types/base.py:453-884``):
@@ -48,6 +50,7 @@ import warnings
 import numpy as np
 import torch
 
+from . import param
 from . import scf as _scf
 from .basis import build_batch
 from .solvation import GeneralizedBorn
@@ -124,7 +127,7 @@ def _check_exclude(val, gfn: str, numbers: np.ndarray) -> None:
     if "disp" not in exc:
         warnings.warn("Dispersion (D3/D4, D4SC) is not computed on the MI355X path: the energy is the reference's "
                       "energy with exclude=['disp'].", UserWarning, stacklevel=4)
-    if gfn == "gfn1" and "hal" not in exc and np.isin(numbers, _HALOGEN_DONORS).any():
+    if param.kind(gfn) == "gfn1" and "hal" not in exc and np.isin(numbers, _HALOGEN_DONORS).any():
         warnings.warn("The GFN1 halogen-bond correction is not computed on the MI355X path: the energy is the "
                       "reference's energy with 'hal' excluded.", UserWarning, stacklevel=4)
 
@@ -422,13 +425,36 @@ class _EnergyFunction(torch.autograd.Function):
         return g.reshape(ctx.shape), None, None, None
 
 
-class GFN1Calculator(_Calculator):
+class Calculator(_Calculator):
+    """Calculator for a GFN1- or GFN2-type parametrization
+    (``calculators/base.py``; constructor ``types/base.py:453-575``):
+    ``Calculator(numbers, par, *, interaction=None, opts=None, device=None,
+    dtype=torch.float64)``.  ``par`` is a ``Param`` (``param.GFN1_XTB``,
+    ``param.GFN2_XTB`` or one read with ``Param.from_file``), its record as a
+    dict, or a parametrization file; its ``meta.name`` selects the GFN1 or
+    GFN2 form (``integral/factory.py:60-83``), every parameter is its own."""
+
+    def __init__(self, numbers: torch.Tensor, par, *, classical=None, interaction=None, opts: dict | None = None,
+                 device=None, dtype: torch.dtype = torch.float64, **kw):
+        self.gfn = param.register(par)
+        self.par = param.get(self.gfn)
+        super().__init__(numbers, classical=classical, interaction=interaction, opts=opts, device=device,
+                         dtype=dtype, **kw)
+
+
+class GFN1Calculator(Calculator):
     """GFN1-xTB on the MI355X path (``calculators/gfn1.py:38-70``)."""
 
-    gfn = "gfn1"
+    def __init__(self, numbers: torch.Tensor, *, classical=None, interaction=None, opts: dict | None = None,
+                 device=None, dtype: torch.dtype = torch.float64, **kw):
+        super().__init__(numbers, "gfn1", classical=classical, interaction=interaction, opts=opts, device=device,
+                         dtype=dtype, **kw)
 
 
-class GFN2Calculator(_Calculator):
+class GFN2Calculator(Calculator):
     """GFN2-xTB on the MI355X path (``calculators/gfn2.py:38-69``)."""
 
-    gfn = "gfn2"
+    def __init__(self, numbers: torch.Tensor, *, classical=None, interaction=None, opts: dict | None = None,
+                 device=None, dtype: torch.dtype = torch.float64, **kw):
+        super().__init__(numbers, "gfn2", classical=classical, interaction=interaction, opts=opts, device=device,
+                         dtype=dtype, **kw)

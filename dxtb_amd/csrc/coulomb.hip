@@ -184,7 +184,7 @@ __global__ __launch_bounds__(POT_BLK) void k_pot_shell(dxtb_basis_t bs, dxtb_par
       const double dx = ra[0] - pb[0], dy = ra[1] - pb[1], dz = ra[2] - pb[2];
       const double R = sqrt(dx * dx + dy * dy + dz * dz);
       double gm, dg;
-      es2_gamma(au == as, R, es2_avg(par.es2_average, hs, par.ush_hub[bs.sh_ush[base + u]]), gm, dg);
+      es2_gamma(au == as, R, es2_avg(par.es2_average, hs, par.ush_hub[bs.sh_ush[base + u]]), par.es2_gexp, gm, dg);
       v += gm * qsh[base + u];
     }
   }
@@ -340,7 +340,7 @@ extern "C" int dxtb_potential(const dxtb_basis_t* bas, const dxtb_params_t* par,
   if (!bas || !par || !numbers || !pos || !nsh || !qsh || !qat || !vsh || !vat || !esh || !eat)
     return DXTB_EINVAL;
   if (par->gfn == 2 && (!mrad || !qdp || !qqp || !vdp || !vqp)) return DXTB_EINVAL;
-  if (par->es2_gexp != 2.0) return DXTB_EUNSUP;
+  if (!(par->es2_gexp > 0.0)) return DXTB_EINVAL;
   const dxtb_basis_t& b = *bas;
   hipLaunchKernelGGL(k_pot_shell, grid1((long)b.nbatch * b.nsh_max * PG, POT_BLK), dim3(POT_BLK), 0, st, b, *par,
                      pos, nsh, qsh, vsh, esh);

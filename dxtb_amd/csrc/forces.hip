@@ -119,7 +119,8 @@ __global__ __launch_bounds__(FBLK) void k_atom_grad(dxtb_basis_t bs, dxtb_params
         const double d[3] = {pi[0] - pu[0], pi[1] - pu[1], pi[2] - pu[2]};
         const double R = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
         double gm, dg;
-        es2_gamma(false, R, es2_avg(par.es2_average, hs, par.ush_hub[bs.sh_ush[b * ns + u]]), gm, dg);
+        es2_gamma(false, R, es2_avg(par.es2_average, hs, par.ush_hub[bs.sh_ush[b * ns + u]]), par.es2_gexp, gm,
+                  dg);
         const double w = qs * qsh[b * ns + u] * dg / R;
         for (int k = 0; k < 3; ++k) g[k] += w * d[k];
       }

@@ -30,6 +30,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from . import param as _param
 from .basis import BatchBasis, build_batch
 from .integrals import MultipoleFunction, multipole_forward
 
@@ -47,6 +48,13 @@ QUADRUPOLE_COMPONENTS = (
 
 
 def _gfn_name(par) -> str:
+    """Parametrization name of ``par``: a Param or its record (registered
+    custom parametrizations included), or a name or object whose meta name
+    is of the GFN1-xTB or GFN2-xTB family."""
+    if isinstance(par, (_param.Param, dict)):
+        return _param.register(par)
+    if isinstance(par, str) and par in _param._REGISTRY:
+        return par
     if isinstance(par, str):
         name = par.casefold()
     else:

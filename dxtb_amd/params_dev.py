@@ -36,7 +36,7 @@ def _hscale(gfn: str, table) -> np.ndarray:
     for u in range(nu):
         within[u] = u - table.first[zs[u]]
     k = np.ones((nu, nu))
-    if gfn == "gfn1":
+    if par.kind == "gfn1":
         val = np.array([par.valence(int(zs[u]))[within[u]] for u in range(nu)])
         kii = np.array([par.h0["kpol"] if not val[u] else keys.get(lab[u] * 2, 1.0) for u in range(nu)])
         for i in range(nu):
@@ -99,7 +99,7 @@ class DeviceParams:
             "z_dkernel": t(zarr("dkernel")), "z_qkernel": t(zarr("qkernel")),
         }
         s = _lib.DxtbParams()
-        s.gfn = 1 if gfn == "gfn1" else 2
+        s.gfn = 1 if par.kind == "gfn1" else 2
         s.es3_shell = 1 if shell_es3 else 0
         s.es2_average = _AVG[par.raw["charge"]["effective"]["average"]]
         s.rep_light = 1 if "klight" in rep else 0

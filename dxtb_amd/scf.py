@@ -266,7 +266,7 @@ class GeometryState:
         self.bb = bb
         self.opts = opts or SCFOptions()
         self.gfn = bb.gfn
-        self.mp = bb.gfn == "gfn2"
+        self.mp = param.kind(bb.gfn) == "gfn2"
         dev = positions.device
         if dev.type != "cuda":
             raise RuntimeError(f"positions must be on a GPU device, not {dev} (there is no CPU path)")

@@ -30,6 +30,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
+from . import param
 from . import scf as hscf
 from .basis import build_batch, shell_table
 
@@ -395,7 +396,7 @@ class Launched:
 class Engine:
     """Continuous-batching engine over the slot pools of one parametrization."""
 
-    gfn: str = "gfn2"
+    gfn: str = "gfn2"  # a parametrization name, Param, record or file
     device: str | torch.device = "cuda:0"
     opts: hscf.SCFOptions = field(default_factory=hscf.SCFOptions)
     capacity: dict | None = None  # class nao -> slots (default: default_capacity)
@@ -410,6 +411,10 @@ class Engine:
     pools: dict = field(default_factory=dict)
     queue: dict = field(default_factory=dict)  # class -> FIFO of requests
     t0: float = field(default_factory=time.perf_counter)
+
+    def __post_init__(self) -> None:
+        # a Param, its record or a file: registered under its own name
+        self.gfn = param.register(self.gfn)
 
     def now(self) -> float:
         return time.perf_counter() - self.t0
@@ -534,7 +539,7 @@ def plain_path(req: Request, gfn: str, device, opts: hscf.SCFOptions | None = No
 
     dev = torch.device(device)
     N = np.asarray(req.numbers, dtype=np.int64)[None]
-    bb = bb_(N, gfn, dev)
+    bb = bb_(N, param.register(gfn), dev)
     st = hscf.GeometryState(bb, torch.as_tensor(np.asarray(req.positions, dtype=np.float64)[None], device=dev),
                             charges=np.array([req.charge]), opts=opts, solvation=[req.solvation],
                             spins=None if req.spin is None else [req.spin])

@@ -90,7 +90,7 @@ typedef struct dxtb_params {
   int es3_shell;           /* 1: shell-resolved third order (GFN2), 0: atom (GFN1) */
   int es2_average;         /* 0 harmonic, 1 arithmetic, 2 geometric */
   int rep_light;           /* 1: light-atom repulsion exponent rep_klight for H/He pairs */
-  double enscale, es2_gexp, rep_kexp, rep_klight;
+  double enscale, es2_gexp, rep_kexp, rep_klight; /* es2_gexp > 0 (2 for GFN1/GFN2) */
   double aes_dmp3, aes_dmp5, aes_kexp, aes_shift, aes_rmax;
   const double* ush_lev;    /* (nush) shell level * EV2AU */
   const double* ush_kcn;    /* (nush) kcn * EV2AU */

@@ -102,7 +102,7 @@ def element_cgtos(gfn: str, z: int):
     for i, sh in enumerate(e["shells"]):
         n, l = params.shell_lab(sh)
         a, c = slater_to_gauss(e["ngauss"][i], n, l, e["slater"][i])
-        if gfn == "gfn1" and not val[i]:
+        if params.kind(gfn) == "gfn1" and not val[i]:
             a, c = orthogonalize((out[i - 1][1], a), (out[i - 1][2], c))
         out.append((l, a, c))
     _CGTO_CACHE[key] = out

@@ -39,7 +39,7 @@ def coordination_number(numbers, positions, gfn: str):
     diff = positions.unsqueeze(-2) - positions.unsqueeze(-3)
     eye = torch.eye(nat, dtype=torch.bool)
     r = torch.sqrt(torch.where(eye, torch.ones_like(rc), (diff * diff).sum(-1)))
-    count = exp_count(r, rc) if gfn == "gfn1" else gfn2_count(r, rc)
+    count = exp_count(r, rc) if params.kind(gfn) == "gfn1" else gfn2_count(r, rc)
     mask = (~eye) & (r <= CN_CUTOFF)
     return torch.where(mask, count, torch.zeros_like(count)).sum(-1)
 
@@ -54,7 +54,7 @@ def hscale(gfn: str, shells) -> torch.Tensor:
     keys = p["shell"]
     n = len(shells)
     k = torch.ones((n, n), dtype=torch.float64)
-    if gfn == "gfn1":
+    if params.kind(gfn) == "gfn1":
         kpol = p["kpol"]
         vals, kii = [], []
         for sh in shells:
@@ -140,7 +140,7 @@ def h0_shell_factor(basis: Basis, positions: torch.Tensor, cn: torch.Tensor) -> 
     var_x = torch.where(diffatom, 1.0 + enscale * (en_sh.unsqueeze(-1) - en_sh.unsqueeze(-2)) ** 2, 0.0)
     kp = torch.tensor([[kpair(gfn, a.z, b.z) for b in shells] for a in shells], dtype=torch.float64)
     hs = hscale(gfn, shells)
-    if gfn == "gfn1":
+    if params.kind(gfn) == "gfn1":
         val = torch.tensor(
             [params.valence(params.elem(gfn, sh.z)["shells"])[sh.ush[1]] for sh in shells]
         )

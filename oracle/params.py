@@ -22,8 +22,35 @@ def load(name: str) -> dict:
         return json.load(fh)
 
 
+#: parametrization records given to the oracle by name (custom ones)
+_CUSTOM: dict = {}
+
+
+def kind(gfn: str) -> str:
+    """Model family of a parametrization name: the part before the first
+    '-' ("gfn1" or "gfn2"; the reference selects the form by the record's
+    meta name, integral/factory.py:74-83)."""
+    k = gfn.split("-", 1)[0]
+    if k not in ("gfn1", "gfn2"):
+        raise ValueError(f"unknown parametrization family in '{gfn}'")
+    return k
+
+
+def register(name: str, data: dict) -> str:
+    """Make a parametrization record known under ``name`` ("gfn1-..." or
+    "gfn2-..."); element fields it leaves out take the defaults of
+    param/element.py:135-205 (0 for gam3, xbond, the multipole kernels and
+    radii)."""
+    kind(name)
+    data = dict(data)
+    data["element"] = {s: {"gam3": 0.0, "xbond": 0.0, "dkernel": 0.0, "qkernel": 0.0, "mprad": 0.0, "mpvcn": 0.0,
+                           **e} for s, e in data["element"].items()}
+    _CUSTOM[name] = data
+    return name
+
+
 def par(gfn: str) -> dict:
-    return load(gfn)
+    return _CUSTOM[gfn] if gfn in _CUSTOM else load(gfn)
 
 
 ELEM = load("elements")

@@ -82,7 +82,7 @@ class System:
         diff = torch.minimum(nuhf, nel)
         nb = (nel - diff) / 2.0
         self.nab = torch.stack([nb + diff, nb])
-        self.is_gfn2 = gfn == "gfn2"
+        self.is_gfn2 = params.kind(gfn) == "gfn2"
 
     # ---- geometry-dependent parts (differentiable in positions) -----------
     def build(self, positions: torch.Tensor):

@@ -61,12 +61,15 @@ int main() {
       h0_poly(R - h, 3.1, ka, kb, pm, t);
       check("h0poly", dp, (pp - pm) / (2 * h), dp);
     }
-    {
+    for (double gexp : {2.0, 1.7, 2.6}) {
       double g0, dg, gp, gm, t;
-      es2_gamma(false, R, 0.45, g0, dg);
-      es2_gamma(false, R + h, 0.45, gp, t);
-      es2_gamma(false, R - h, 0.45, gm, t);
+      es2_gamma(false, R, 0.45, gexp, g0, dg);
+      es2_gamma(false, R + h, 0.45, gexp, gp, t);
+      es2_gamma(false, R - h, 0.45, gexp, gm, t);
       check("gamma", dg, (gp - gm) / (2 * h), dg);
+      // the closed form itself: ((R + eps)^g + avg^-g)^(-1/g)
+      const double ref = std::pow(std::pow(R + EPS, gexp) + std::pow(0.45, -gexp), -1.0 / gexp);
+      check("gamma value", g0, ref, ref);
     }
     {
       double e, de, ep, em, t;
