@@ -1133,25 +1133,39 @@ __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const JArgs* _
 
 constexpr int SPLIT_MIN_N = 128;  // auto: split form from this order, for batches below half of it
 
+// VT = 1 and the split state cleared: grid (matrix, chunk of INIT_ROWS rows),
+// so that a lone matrix's identity is written by many workgroups at once
+constexpr int INIT_ROWS = 8;
+__host__ __device__ constexpr int init_chunks(int nmax) { return (nmax + INIT_ROWS - 1) / INIT_ROWS; }
+
 __global__ __launch_bounds__(JBB) void k_bj_init(int nmax, const JArgs* __restrict__ ja, double* __restrict__ sst) {
-  const int b = blockIdx.x;
+  const int b = blockIdx.x, c = blockIdx.y;
   const JArgsC jc = jconst(ja);
   if (b >= jc->nbatch) return;
   double* Vb = jc->VT + (size_t)b * nmax * nmax;
-  for (int e = threadIdx.x; e < nmax * nmax; e += JBB) {
+  const int r0 = c * INIT_ROWS, r1 = min(nmax, r0 + INIT_ROWS);
+  const int e0 = r0 * nmax, e1 = r1 * nmax;
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += JBB) {
     const int i = e / nmax, j = e - i * nmax;
     Vb[e] = (i == j) ? 1.0 : 0.0;
   }
-  if (threadIdx.x < 4) sst[4 * b + threadIdx.x] = 0.0;
+  if (c == 0 && threadIdx.x < 4) sst[4 * b + threadIdx.x] = 0.0;
 }
 
 // Convergence test of the split form, in two launches: k_bj_norms reduces
-// chunks of about 16384 elements (whole rows) of every matrix over many
-// workgroups, k_bj_check combines a matrix's chunks and decides whether it
-// stops.  (One workgroup reading a whole matrix took 2.2 ms at order 4002.)
-__host__ __device__ constexpr int norm_rows(int nmax) { return nmax >= 16384 ? 1 : 16384 / nmax; }
-__host__ __device__ constexpr int norm_chunks(int nmax) { return (nmax + norm_rows(nmax) - 1) / norm_rows(nmax); }
+// chunks of whole rows of every matrix over many workgroups, k_bj_check
+// combines a matrix's chunks and decides whether it stops.  (One workgroup
+// reading a whole matrix took 2.2 ms at order 4002.)  A chunk holds about
+// NORM_EL elements, at most 1024 chunks per matrix; every thread keeps NPT
+// loads in flight at a time (a lone order-240 matrix: 25 us with one load at
+// a time over chunks of 16384 elements).
 constexpr int NRB = 256;  // threads per k_bj_norms workgroup
+constexpr int NPT = 8;    // loads in flight per thread
+constexpr int NORM_EL = NRB * NPT;
+__host__ __device__ constexpr int norm_rows(int nmax) {
+  return (NORM_EL / nmax > (nmax + 1023) / 1024) ? NORM_EL / nmax : (nmax + 1023) / 1024;
+}
+__host__ __device__ constexpr int norm_chunks(int nmax) { return (nmax + norm_rows(nmax) - 1) / norm_rows(nmax); }
 
 __global__ __launch_bounds__(NRB) void k_bj_norms(int nmax, const JArgs* __restrict__ ja,
                                                  const double* __restrict__ sst, double* __restrict__ nrm) {
@@ -1164,11 +1178,24 @@ __global__ __launch_bounds__(NRB) void k_bj_norms(int nmax, const JArgs* __restr
   const int r0 = c * norm_rows(nmax), r1 = min(n, r0 + norm_rows(nmax));
   double off = 0.0, amax = 0.0;
   const int ncol = n, nel = max(r1 - r0, 0) * ncol;
-  for (int e = threadIdx.x; e < nel; e += NRB) {
-    const int i = r0 + e / ncol, j = e - (e / ncol) * ncol;
-    const double x = fabs(A[(size_t)i * nmax + j]);
-    off = fmax(off, i == j ? 0.0 : x);
-    amax = fmax(amax, i == j ? x : 0.0);
+  for (int base = threadIdx.x; base < nel; base += NRB * NPT) {
+    // element indices past the chunk are clamped to its last element: read
+    // again, they leave the maxima unchanged
+    double x[NPT];
+    bool dg[NPT];
+#pragma unroll
+    for (int u = 0; u < NPT; ++u) {
+      const int e = min(base + u * NRB, nel - 1);
+      const int q = e / ncol;
+      const int i = r0 + q, j = e - q * ncol;
+      x[u] = fabs(A[(size_t)i * nmax + j]);
+      dg[u] = i == j;
+    }
+#pragma unroll
+    for (int u = 0; u < NPT; ++u) {
+      off = fmax(off, dg[u] ? 0.0 : x[u]);
+      amax = fmax(amax, dg[u] ? x[u] : 0.0);
+    }
   }
   off = block_max<NRB / 64>(off, red);
   amax = block_max<NRB / 64>(amax, red);
@@ -1187,9 +1214,19 @@ __global__ __launch_bounds__(64) void k_bj_check(int nmax, const JArgs* __restri
   const double* __restrict__ tol_b = jc->tol_b;
   const int nch = norm_chunks(nmax);
   double off = 0.0, amax = 0.0;
-  for (int c = lane; c < nch; c += 64) {
-    off = fmax(off, nrm[((size_t)b * nch + c) * 2]);
-    amax = fmax(amax, nrm[((size_t)b * nch + c) * 2 + 1]);
+  for (int base = lane; base < nch; base += 64 * 4) {
+    double o[4], a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {  // clamped to the last chunk, as in k_bj_norms
+      const int c = min(base + 64 * u, nch - 1);
+      o[u] = nrm[((size_t)b * nch + c) * 2];
+      a[u] = nrm[((size_t)b * nch + c) * 2 + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      off = fmax(off, o[u]);
+      amax = fmax(amax, a[u]);
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -1355,7 +1392,8 @@ hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax,
   const bool defer = defer_call(nmax, nbcap);
   const int nwork = npairs * (npairs - 1) / 2 + (defer ? 0 : npairs * npairs);
   if (nsw > 0) {
-    hipLaunchKernelGGL(k_bj_init, dim3((unsigned)nbcap), dim3(JBB), 0, st, nmax, w.ja, w.sst);
+    hipLaunchKernelGGL(k_bj_init, dim3((unsigned)nbcap, (unsigned)init_chunks(nmax)), dim3(JBB), 0, st, nmax, w.ja,
+                       w.sst);
     for (int sw = 0; sw < nsw; ++sw) {
       hipLaunchKernelGGL(k_bj_norms, dim3((unsigned)nbcap, (unsigned)norm_chunks(nmax)), dim3(NRB), 0, st, nmax,
                          w.ja, w.sst, w.nrm);

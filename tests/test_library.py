@@ -53,12 +53,13 @@ def test_jacobi_arguments_without_gpu():
     # whole sweep's are kept up to order 512), 4 doubles of split-sweep state,
     # per step 128 int rotation flags (block pairs of the largest order, 4096;
     # the flags of the batch rounded up to 256 bytes), and two doubles per
-    # chunk of rows of the convergence test (about 16384 elements per chunk)
+    # chunk of rows of the convergence test (2048 // n rows, at least enough
+    # for at most 1024 chunks)
     assert lib.dxtb_jacobi_max_n() == 4096
 
     def size(cap, n, steps, npad):
         flags = -(-cap * steps * 128 * 4 // 256) * 256
-        chunks = -(-n // max(1, 16384 // n))
+        chunks = -(-n // max(2048 // n, -(-n // 1024)))
         return 256 + cap * steps * npad * 32 * 8 + cap * 4 * 8 + flags + cap * chunks * 2 * 8
 
     assert lib.dxtb_jacobi_work_size(2, 33) == size(2, 33, 3, 64)
@@ -67,9 +68,10 @@ def test_jacobi_arguments_without_gpu():
         assert lib.dxtb_jacobi_work_size(nb, 33) == size(cap, 33, 3, 64), nb
     assert lib.dxtb_jacobi_work_size(0, 33) == 0
     # above order 512 the rotations of one step only: order 600, 608 rows,
-    # 23 chunks of 27 rows
+    # 200 chunks of 3 rows; order 4000: 1000 chunks of 4 rows
     assert lib.dxtb_jacobi_work_size(1, 600) == size(1, 600, 1, 608)
-    assert size(1, 600, 1, 608) == 256 + 608 * 32 * 8 + 4 * 8 + 512 + 23 * 16
+    assert size(1, 600, 1, 608) == 256 + 608 * 32 * 8 + 4 * 8 + 512 + 200 * 16
+    assert lib.dxtb_jacobi_work_size(1, 4000) == size(1, 4000, 1, 4000)
     assert lib.dxtb_jacobi_max_n() >= 128
     args = (None,) * 5
     assert lib.dxtb_jacobi_eigh(0, 10, *args, 0, 10, 0, 1e-14, None, None, None) == _lib.DXTB_EINVAL
