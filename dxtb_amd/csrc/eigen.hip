@@ -472,6 +472,33 @@ __device__ __forceinline__ void block_pair(int k, int s, int nb, int& bi, int& b
   bj = max(a, c);
 }
 
+// slot of block x at outer step s (player's inverse), its block pair, and
+// the block it is paired with
+__device__ __forceinline__ int block_slot(int x, int s, int nb) {
+  if (x == 0) return 0;
+  int t = (x - 1 - s) % (nb - 1);
+  if (t < 0) t += nb - 1;
+  return t + 1;
+}
+__device__ __forceinline__ int pair_of_block(int x, int s, int nb) {
+  const int sl = block_slot(x, s, nb);
+  return sl < nb / 2 ? sl : nb - 1 - sl;
+}
+__device__ __forceinline__ int partner_block(int x, int s, int nb) {
+  return player(nb - 1 - block_slot(x, s, nb), s, nb);
+}
+
+// Whether the off-diagonal tile (k, k2) of step s - 1 holds the block A(I, J)
+// of a block pair (I, J) of step s.  Every pair of step s straddles two pairs
+// of step s - 1 (the circle ordering pairs no two blocks twice in a sweep),
+// so it lies in exactly one such tile; above four blocks no tile holds two.
+__device__ __forceinline__ bool tile_holds_next_pair(int nb, int s, int k, int k2) {
+  int bi, bj;
+  block_pair(k, s - 1, nb, bi, bj);
+  return pair_of_block(partner_block(bi, s, nb), s - 1, nb) == k2 ||
+         pair_of_block(partner_block(bj, s, nb), s - 1, nb) == k2;
+}
+
 // Deferred eigenvector update: up to this order the rotations of a whole
 // outer sweep are kept (a log of (nb - 1) steps x npairs 32x32 blocks per
 // matrix) and applied to VT once per sweep, a 32-column panel of all rows at
@@ -488,9 +515,11 @@ __host__ __device__ constexpr bool defer_vt(int nmax) { return nmax <= DEFER_MAX
 __host__ __device__ constexpr bool defer_call(int nmax, int nbcap) {
   return defer_vt(nmax) && nbcap > DEFER_MIN_BATCH;
 }
-// steps of rotations kept per matrix: a sweep's worth when deferred, else one
+// steps of rotations kept per matrix: a sweep's worth up to DEFER_MAX
+// (else two, used by step parity: a launch that forms step s - 1's products
+// and step s's rotations reads the one and writes the other)
 __host__ __device__ constexpr int log_steps(int nmax) {
-  return defer_vt(nmax) ? 2 * ((nmax + SB - 1) / SB) - 1 : 1;
+  return defer_vt(nmax) ? 2 * ((nmax + SB - 1) / SB) - 1 : 2;
 }
 
 // One matrix of the blocked variant.
@@ -522,11 +551,12 @@ __device__ __forceinline__ BjMat bj_mat(int b, int nmax, const int* nsize, doubl
   return M;
 }
 
-// rotations and flags of outer step s
+// rotations and flags of outer step s: a slot per step when deferred, else
+// by step parity
 __device__ __forceinline__ double* qstep(const BjMat& M, int s) {
-  return M.Qlog + (size_t)(M.defer ? s : 0) * M.npad * SB;
+  return M.Qlog + (size_t)(M.defer ? s : (s & 1)) * M.npad * SB;
 }
-__device__ __forceinline__ int* fstep(const BjMat& M, int s) { return M.flags + (M.defer ? s : 0) * KMAX; }
+__device__ __forceinline__ int* fstep(const BjMat& M, int s) { return M.flags + (M.defer ? s : (s & 1)) * KMAX; }
 
 // Data pointers and batch size of one dxtb_jacobi_eigh call, written into the
 // head of the workspace by k_bj_args before the call's other kernels run.
@@ -883,83 +913,99 @@ __device__ __forceinline__ int stile(int r, int c) { return r * SB + (c ^ (r & (
 template <bool SWZ>
 __device__ __forceinline__ int tix(int r, int c) { return SWZ ? stile(r, c) : r * SST + c; }
 
-// Product item `item` of outer step s, by one wavefront (Sx: its staging tile
-// in LDS, indexed by tix<SWZ>): an off-diagonal tile (k < k2) becomes
-// Q_k^T A_kk2 Q_k2, a 32-column panel of the VT rows of pair k becomes
-// Q_k^T VT.  rot: flags of the pairs.
+// tile index -> (k, k2), k < k2
+__device__ __forceinline__ void tile_pairs(int item, int npairs, int& k, int& k2) {
+  int rem = item;
+  k = 0;
+  while (rem >= npairs - 1 - k) {
+    rem -= npairs - 1 - k;
+    ++k;
+  }
+  k2 = k + 1 + rem;
+}
+
+// The off-diagonal tile (k < k2) of outer step s becomes Q_k^T A_kk2 Q_k2, by
+// one wavefront (Sx: its staging tile in LDS, indexed by tix<SWZ>), written
+// to both triangles.  rot: flags of the pairs.
+template <bool SWZ>
+__device__ void bj_product_tile(double* __restrict__ Sx, const BjMat& M, const int* rot, int k, int k2, int s,
+                                int lane) {
+  const int n = M.n, nmax = M.nmax, nlast = M.nlast;
+  const bool rk = rot[k] != 0, rk2 = rot[k2] != 0;
+  if (!rk && !rk2) return;
+  int bi, bj, ci, cj;
+  block_pair(k, s, M.nb, bi, bj);
+  block_pair(k2, s, M.nb, ci, cj);
+  double* A = M.A;
+  auto loadA = [&](int m, int c) -> double {
+    const int gr = gidx(m, bi, bj), gc = gidx(c, ci, cj);
+    const double x = A[(size_t)min(gr, nlast) * nmax + min(gc, nlast)];
+    return x * mask01(gr < n && gc < n);
+  };
+  d4_t acc[2][2];
+  if (rk) {
+    qt_times(qstep(M, s) + (size_t)k * SB * SB, loadA, lane, acc);
+  } else {  // Q_k = 1: T is the tile itself
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[rb][cb][i] = loadA(rb * BS + acc_row(lane, i), cb * BS + (lane & 15));
+  }
+  // T = Q_k^T X -> LDS (swizzled rows)
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        Sx[tix<SWZ>(rb * BS + acc_row(lane, i), cb * BS + (lane & 15))] = acc[rb][cb][i];
+  wave_sync();
+  if (rk2) {
+    // U^T = Q_k2^T T^T with U = T Q_k2: the same product, T read transposed
+    auto loadT = [&](int m, int c) -> double { return Sx[tix<SWZ>(c, m)]; };
+    qt_times(qstep(M, s) + (size_t)k2 * SB * SB, loadT, lane, acc);
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int uc = rb * BS + acc_row(lane, i), ur = cb * BS + (lane & 15);
+          const int gr = gidx(ur, bi, bj), gc = gidx(uc, ci, cj);
+          if (gr < n && gc < n) {
+            const double v = acc[rb][cb][i];
+            A[(size_t)gr * nmax + gc] = v;
+            A[(size_t)gc * nmax + gr] = v;
+          }
+        }
+  } else {  // Q_k2 = 1: U = T
+    for (int e = lane; e < SB * SB; e += 64) {
+      const int r = e >> 5, c = e & 31;
+      const int gr = gidx(r, bi, bj), gc = gidx(c, ci, cj);
+      if (gr < n && gc < n) {
+        const double v = Sx[tix<SWZ>(r, c)];
+        A[(size_t)gr * nmax + gc] = v;
+        A[(size_t)gc * nmax + gr] = v;
+      }
+    }
+  }
+  wave_sync();
+}
+
+// Product item `item` of outer step s, by one wavefront: an off-diagonal tile
+// (bj_product_tile), or a 32-column panel of the VT rows of pair k, which
+// becomes Q_k^T VT.
 template <bool SWZ>
 __device__ void bj_product_item(double* __restrict__ Sx, const BjMat& M, const int* rot, int item, int s, int lane) {
   const int npairs = M.npairs, n = M.n, nmax = M.nmax, nlast = M.nlast;
   const int ntile = npairs * (npairs - 1) / 2;
   const int npanel = (n + SB - 1) / SB;
   if (item < ntile) {
-    int k = 0, rem = item;  // tile index -> (k, k2), k < k2
-    while (rem >= npairs - 1 - k) {
-      rem -= npairs - 1 - k;
-      ++k;
-    }
-    const int k2 = k + 1 + rem;
-    const bool rk = rot[k] != 0, rk2 = rot[k2] != 0;
-    if (!rk && !rk2) return;
-    int bi, bj, ci, cj;
-    block_pair(k, s, M.nb, bi, bj);
-    block_pair(k2, s, M.nb, ci, cj);
-    double* A = M.A;
-    auto loadA = [&](int m, int c) -> double {
-      const int gr = gidx(m, bi, bj), gc = gidx(c, ci, cj);
-      const double x = A[(size_t)min(gr, nlast) * nmax + min(gc, nlast)];
-      return x * mask01(gr < n && gc < n);
-    };
-    d4_t acc[2][2];
-    if (rk) {
-      qt_times(qstep(M, s) + (size_t)k * SB * SB, loadA, lane, acc);
-    } else {  // Q_k = 1: T is the tile itself
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) acc[rb][cb][i] = loadA(rb * BS + acc_row(lane, i), cb * BS + (lane & 15));
-    }
-    // T = Q_k^T X -> LDS (swizzled rows)
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-      for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          Sx[tix<SWZ>(rb * BS + acc_row(lane, i), cb * BS + (lane & 15))] = acc[rb][cb][i];
-    wave_sync();
-    if (rk2) {
-      // U^T = Q_k2^T T^T with U = T Q_k2: the same product, T read transposed
-      auto loadT = [&](int m, int c) -> double { return Sx[tix<SWZ>(c, m)]; };
-      qt_times(qstep(M, s) + (size_t)k2 * SB * SB, loadT, lane, acc);
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int uc = rb * BS + acc_row(lane, i), ur = cb * BS + (lane & 15);
-            const int gr = gidx(ur, bi, bj), gc = gidx(uc, ci, cj);
-            if (gr < n && gc < n) {
-              const double v = acc[rb][cb][i];
-              A[(size_t)gr * nmax + gc] = v;
-              A[(size_t)gc * nmax + gr] = v;
-            }
-          }
-    } else {  // Q_k2 = 1: U = T
-      for (int e = lane; e < SB * SB; e += 64) {
-        const int r = e >> 5, c = e & 31;
-        const int gr = gidx(r, bi, bj), gc = gidx(c, ci, cj);
-        if (gr < n && gc < n) {
-          const double v = Sx[tix<SWZ>(r, c)];
-          A[(size_t)gr * nmax + gc] = v;
-          A[(size_t)gc * nmax + gr] = v;
-        }
-      }
-    }
-    wave_sync();
+    int k, k2;
+    tile_pairs(item, npairs, k, k2);
+    bj_product_tile<SWZ>(Sx, M, rot, k, k2, s, lane);
   } else {
     const int w2 = item - ntile;
     const int k = w2 / npanel, pc = w2 - k * npanel;
@@ -1284,21 +1330,6 @@ __global__ __launch_bounds__(CW * 64) void k_bj_inner_full(int nmax, const JArgs
   if (tid == 0) fstep(M, 0)[k] = any ? 1 : 0;
 }
 
-// Inner phase of the split form at outer steps s > 0: one CW-wavefront
-// workgroup per (block pair, matrix).
-__global__ __launch_bounds__(CW * 64) void k_bj_inner_cross(int nmax, const JArgs* __restrict__ ja,
-                                                           double* __restrict__ work, const double* __restrict__ sst,
-                                                           int* __restrict__ rot, int s) {
-  __shared__ CrossLds X;
-  const int b = blockIdx.y, k = blockIdx.x;
-  const JArgsC jc = jconst(ja);
-  if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
-  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, rot, jc->defer);
-  if (k >= M.npairs || s >= M.nb - 1) return;
-  const bool any = bj_inner_cross4(X, M, k, s, sst[4 * b + 1], threadIdx.x);
-  if (threadIdx.x == 0) fstep(M, s)[k] = any ? 1 : 0;
-}
-
 __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(5))) void k_bj_product(int nmax, const JArgs* __restrict__ ja, double* __restrict__ work,
                                                    const double* __restrict__ sst, int* __restrict__ rot,
                                                    int s) {
@@ -1315,6 +1346,62 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(5))) vo
   const int item = blockIdx.y * PW + wv;
   if (item < bj_nwork(M)) bj_product_item<true>(lds_raw + (size_t)wv * PROD_TILE, M, fstep(M, s), item, s, lane);
 }
+
+// Outer step s > 0 of the split form in one launch: the product phase of
+// step s - 1 and the cross sweeps of step s.  Workgroup y < npairs(nmax)
+// forms the tile of step s - 1 that holds A(I, J) of block pair y = (I, J) of
+// step s (wavefront 0, bj_product_tile), then sweeps the pair with all four
+// wavefronts (bj_inner_cross4: the new tile, and the diagonal blocks A(I, I),
+// A(J, J), which the sweeps of step s - 1 wrote and no product changes); the
+// workgroups above take the other product items of step s - 1, PW per
+// workgroup, as k_bj_product.  No workgroup writes what another reads: a
+// pair's sweep writes its own blocks (in its own tile and in diagonal tiles
+// of step s - 1, which no product reads) and the rotations and flags of step
+// s (kept apart from those of step s - 1).  With four blocks the only tile
+// holds both pairs: the workgroup of pair 0 sweeps them one after the other.
+// A matrix whose sweep ends before step s only forms its last products.
+// Same arithmetic as the two launches it replaces, bit for bit.
+__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(5))) void k_bj_step(
+    int nmax, const JArgs* __restrict__ ja, double* __restrict__ work, const double* __restrict__ sst,
+    int* __restrict__ rot, int s) {
+  extern __shared__ double lds_raw[];  // staging tiles; a sweep's exchange area after the first
+  const int b = blockIdx.x;            // the matrix: its workgroups on one XCD, as k_bj_product
+  const JArgsC jc = jconst(ja);
+  if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
+  const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, rot, jc->defer);
+  if (s < 1 || s > M.nb - 1) return;
+  const bool sweep = s < M.nb - 1;  // step s exists for this matrix
+  const int npmax = (nmax + SB - 1) / SB;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int y = blockIdx.y;
+  if (y < npmax) {
+    if (!sweep || y >= M.npairs || (M.nb == 4 && y == 1)) return;
+    int bi, bj;
+    block_pair(y, s, M.nb, bi, bj);
+    const int ka = pair_of_block(bi, s - 1, M.nb), kb = pair_of_block(bj, s - 1, M.nb);
+    if (wv == 0) bj_product_tile<true>(lds_raw, M, fstep(M, s - 1), min(ka, kb), max(ka, kb), s - 1, lane);
+    __syncthreads();  // the tile's writes to A seen by every wavefront
+    CrossLds& X = *reinterpret_cast<CrossLds*>(lds_raw + PROD_TILE);
+    const double thr = sst[4 * b + 1];
+    const int klast = M.nb == 4 ? 1 : y;
+    for (int k = y; k <= klast; ++k) {
+      const bool any = bj_inner_cross4(X, M, k, s, thr, threadIdx.x);
+      if (threadIdx.x == 0) fstep(M, s)[k] = any ? 1 : 0;
+      __syncthreads();  // X is reused by the next pair
+    }
+    return;
+  }
+  const int item = (y - npmax) * PW + wv;
+  if (item >= bj_nwork(M)) return;
+  if (sweep && item < M.npairs * (M.npairs - 1) / 2) {
+    int k, k2;
+    tile_pairs(item, M.npairs, k, k2);
+    if (tile_holds_next_pair(M.nb, s, k, k2)) return;  // formed by the workgroup of that pair
+  }
+  bj_product_item<true>(lds_raw + (size_t)wv * PROD_TILE, M, fstep(M, s - 1), item, s - 1, lane);
+}
+static_assert((size_t)PROD_TILE * sizeof(double) + sizeof(CrossLds) <= PROD_LDS,
+              "k_bj_step: a staging tile and the sweep's exchange area in the product LDS");
 
 // Deferred eigenvector update of one sweep: one workgroup per (matrix, VT
 // panel of 32 columns), the panel's rows in LDS.
@@ -1398,19 +1485,19 @@ hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax,
       hipLaunchKernelGGL(k_bj_norms, dim3((unsigned)nbcap, (unsigned)norm_chunks(nmax)), dim3(NRB), 0, st, nmax,
                          w.ja, w.sst, w.nrm);
       hipLaunchKernelGGL(k_bj_check, dim3((unsigned)nbcap), dim3(64), 0, st, nmax, w.ja, w.nrm, tol, w.sst, sw);
-      for (int s = 0; s < nb - 1; ++s) {
-        if (s == 0)  // the full sweep of every subproblem (from LDS), then the cross sweeps
-          hipLaunchKernelGGL(k_bj_inner_full, dim3((unsigned)npairs, (unsigned)nbcap), dim3(CW * 64), 0, st, nmax,
-                             w.ja, w.rot_q, w.sst, w.rot);
-        else
-          hipLaunchKernelGGL(k_bj_inner_cross, dim3((unsigned)npairs, (unsigned)nbcap), dim3(CW * 64), 0, st, nmax,
-                             w.ja, w.rot_q, w.sst, w.rot, s);
-        if (nwork > 0)  // one block pair with deferred VT: nothing to multiply
-          hipLaunchKernelGGL(k_bj_product, dim3((unsigned)nbcap, (unsigned)((nwork + PW - 1) / PW)), dim3(PW * 64),
-                             PROD_LDS, st, nmax, w.ja, w.rot_q, w.sst, w.rot, s);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-      }
+      // step 0: the full sweep of every subproblem (from LDS); steps 1 ..
+      // nb - 2: the products of the step before and the cross sweeps in one
+      // launch; then the products of the last step
+      hipLaunchKernelGGL(k_bj_inner_full, dim3((unsigned)npairs, (unsigned)nbcap), dim3(CW * 64), 0, st, nmax, w.ja,
+                         w.rot_q, w.sst, w.rot);
+      for (int s = 1; s < nb - 1; ++s)
+        hipLaunchKernelGGL(k_bj_step, dim3((unsigned)nbcap, (unsigned)(npairs + (nwork + PW - 1) / PW)),
+                           dim3(PW * 64), PROD_LDS, st, nmax, w.ja, w.rot_q, w.sst, w.rot, s);
+      if (nwork > 0)  // one block pair with deferred VT: nothing to multiply
+        hipLaunchKernelGGL(k_bj_product, dim3((unsigned)nbcap, (unsigned)((nwork + PW - 1) / PW)), dim3(PW * 64),
+                           PROD_LDS, st, nmax, w.ja, w.rot_q, w.sst, w.rot, nb - 2);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
       if (defer)
         hipLaunchKernelGGL(k_bj_apply, dim3((unsigned)nbcap, (unsigned)npairs), dim3(AW * 64), apply_lds(nmax), st,
                            nmax, w.ja, w.rot_q, w.sst, w.rot);

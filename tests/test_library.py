@@ -50,7 +50,8 @@ def test_jacobi_arguments_without_gpu():
     # workspace: a 256-byte argument block, then per matrix of the batch
     # capacity the 32x32 rotations of the order padded to a multiple of 32 for
     # every step of a sweep (order 33: 2 * 2 - 1 = 3 steps of 64 rows; the
-    # whole sweep's are kept up to order 512), 4 doubles of split-sweep state,
+    # whole sweep's are kept up to order 512, two steps above), 4 doubles of
+    # split-sweep state,
     # per step 128 int rotation flags (block pairs of the largest order, 4096;
     # the flags of the batch rounded up to 256 bytes), and two doubles per
     # chunk of rows of the convergence test (2048 // n rows, at least enough
@@ -67,11 +68,11 @@ def test_jacobi_arguments_without_gpu():
     for nb, cap in ((1, 1), (3, 4), (5, 8), (9, 16), (63, 64), (65, 96), (256, 256), (257, 288)):
         assert lib.dxtb_jacobi_work_size(nb, 33) == size(cap, 33, 3, 64), nb
     assert lib.dxtb_jacobi_work_size(0, 33) == 0
-    # above order 512 the rotations of one step only: order 600, 608 rows,
-    # 200 chunks of 3 rows; order 4000: 1000 chunks of 4 rows
-    assert lib.dxtb_jacobi_work_size(1, 600) == size(1, 600, 1, 608)
-    assert size(1, 600, 1, 608) == 256 + 608 * 32 * 8 + 4 * 8 + 512 + 200 * 16
-    assert lib.dxtb_jacobi_work_size(1, 4000) == size(1, 4000, 1, 4000)
+    # above order 512 the rotations of two steps only (by step parity): order
+    # 600, 608 rows, 200 chunks of 3 rows; order 4000: 1000 chunks of 4 rows
+    assert lib.dxtb_jacobi_work_size(1, 600) == size(1, 600, 2, 608)
+    assert size(1, 600, 2, 608) == 256 + 2 * 608 * 32 * 8 + 4 * 8 + 1024 + 200 * 16
+    assert lib.dxtb_jacobi_work_size(1, 4000) == size(1, 4000, 2, 4000)
     assert lib.dxtb_jacobi_max_n() >= 128
     args = (None,) * 5
     assert lib.dxtb_jacobi_eigh(0, 10, *args, 0, 10, 0, 1e-14, None, None, None) == _lib.DXTB_EINVAL
