@@ -157,6 +157,7 @@ __device__ __forceinline__ double group_sum(double v) {
 }
 
 // ---- shell potential: ES2 gamma*q (+ GFN2 shell ES3) ----------------------
+template <bool G2>
 __global__ __launch_bounds__(POT_BLK) void k_pot_shell(dxtb_basis_t bs, dxtb_params_t par,
                                                        const double* __restrict__ pos, const int* __restrict__ nsh,
                                                        const double* __restrict__ qsh, double* __restrict__ vsh,
@@ -184,7 +185,8 @@ __global__ __launch_bounds__(POT_BLK) void k_pot_shell(dxtb_basis_t bs, dxtb_par
       const double dx = ra[0] - pb[0], dy = ra[1] - pb[1], dz = ra[2] - pb[2];
       const double R = sqrt(dx * dx + dy * dy + dz * dz);
       double gm, dg;
-      es2_gamma(au == as, R, es2_avg(par.es2_average, hs, par.ush_hub[bs.sh_ush[base + u]]), par.es2_gexp, gm, dg);
+      es2_gamma_t<G2>(au == as, R, es2_avg(par.es2_average, hs, par.ush_hub[bs.sh_ush[base + u]]), par.es2_gexp, gm,
+                      dg);
       v += gm * qsh[base + u];
     }
   }
@@ -342,8 +344,12 @@ extern "C" int dxtb_potential(const dxtb_basis_t* bas, const dxtb_params_t* par,
   if (par->gfn == 2 && (!mrad || !qdp || !qqp || !vdp || !vqp)) return DXTB_EINVAL;
   if (!(par->es2_gexp > 0.0)) return DXTB_EINVAL;
   const dxtb_basis_t& b = *bas;
-  hipLaunchKernelGGL(k_pot_shell, grid1((long)b.nbatch * b.nsh_max * PG, POT_BLK), dim3(POT_BLK), 0, st, b, *par,
-                     pos, nsh, qsh, vsh, esh);
+  if (par->es2_gexp == 2.0)
+    hipLaunchKernelGGL(k_pot_shell<true>, grid1((long)b.nbatch * b.nsh_max * PG, POT_BLK), dim3(POT_BLK), 0, st, b,
+                       *par, pos, nsh, qsh, vsh, esh);
+  else
+    hipLaunchKernelGGL(k_pot_shell<false>, grid1((long)b.nbatch * b.nsh_max * PG, POT_BLK), dim3(POT_BLK), 0, st, b,
+                       *par, pos, nsh, qsh, vsh, esh);
   DXTB_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_pot_atom, grid1((long)b.nbatch * b.nat_max * PG, POT_BLK), dim3(POT_BLK), 0, st, b.nbatch,
                      b.nat_max, *par, numbers, pos, mrad, qat, qdp, qqp, vat, vdp, vqp, eat);

@@ -88,6 +88,7 @@ __global__ __launch_bounds__(FBLK) void k_h0_grad(dxtb_basis_t bs, const int* __
 }
 
 // per atom: ES2, AES2, repulsion gradients and dE/dCN (H0 self-energies + AES2 radii)
+template <bool G2>
 __global__ __launch_bounds__(FBLK) void k_atom_grad(dxtb_basis_t bs, dxtb_params_t par,
                                                     const int* __restrict__ numbers, const double* __restrict__ pos,
                                                     const int* __restrict__ at_sh0, const int* __restrict__ at_nsh,
@@ -119,8 +120,8 @@ __global__ __launch_bounds__(FBLK) void k_atom_grad(dxtb_basis_t bs, dxtb_params
         const double d[3] = {pi[0] - pu[0], pi[1] - pu[1], pi[2] - pu[2]};
         const double R = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
         double gm, dg;
-        es2_gamma(false, R, es2_avg(par.es2_average, hs, par.ush_hub[bs.sh_ush[b * ns + u]]), par.es2_gexp, gm,
-                  dg);
+        es2_gamma_t<G2>(false, R, es2_avg(par.es2_average, hs, par.ush_hub[bs.sh_ush[b * ns + u]]), par.es2_gexp,
+                        gm, dg);
         const double w = qs * qsh[b * ns + u] * dg / R;
         for (int k = 0; k < 3; ++k) g[k] += w * d[k];
       }
@@ -224,8 +225,12 @@ extern "C" int dxtb_forces_classical(const dxtb_basis_t* bas, const dxtb_params_
   hipLaunchKernelGGL(k_h0_grad, grid1((long)b.nbatch * b.nsh_max * FG, FBLK), dim3(FBLK), 0, st, b, nsh, pos, P,
                      S, dfsh, hfsh, dEdh, grad);
   DXTB_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_atom_grad, grid1((long)b.nbatch * b.nat_max * FG, FBLK), dim3(FBLK), 0, st, b, *par,
-                     numbers, pos, at_sh0, at_nsh, nsh, dEdh, cn, mrad, qsh, qat, qdp, qqp, dEdcn, grad);
+  if (par->es2_gexp == 2.0)
+    hipLaunchKernelGGL(k_atom_grad<true>, grid1((long)b.nbatch * b.nat_max * FG, FBLK), dim3(FBLK), 0, st, b, *par,
+                       numbers, pos, at_sh0, at_nsh, nsh, dEdh, cn, mrad, qsh, qat, qdp, qqp, dEdcn, grad);
+  else
+    hipLaunchKernelGGL(k_atom_grad<false>, grid1((long)b.nbatch * b.nat_max * FG, FBLK), dim3(FBLK), 0, st, b, *par,
+                       numbers, pos, at_sh0, at_nsh, nsh, dEdh, cn, mrad, qsh, qat, qdp, qqp, dEdcn, grad);
   DXTB_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_cn_grad, grid1((long)b.nbatch * b.nat_max * FG, FBLK), dim3(FBLK), 0, st, b.nbatch,
                      b.nat_max, par->gfn, numbers, pos, par->z_rcov, dEdcn, grad);

@@ -45,9 +45,8 @@ DXTB_HD void h0_poly(double R, double rab, double ka, double kb, double& p, doub
   dp = (ka * (1.0 + kb * sr) + kb * (1.0 + ka * sr)) * 0.5 / sqrt(R * rab);
 }
 
-// ES2 shell pair: gamma = ((R+eps)^g + avg^-g)^(-1/g) and its R derivative;
-// same atom: (eps + avg^-g)^(-1/g) (secondorder.py:799-870).  g = 2, the
-// exponent of GFN1 and GFN2, is evaluated without pow.
+// ES2 shell pair: gamma = ((R+eps)^2 + avg^-2)^(-1/2) for gexp = 2 (GFN1 and
+// GFN2) and its R derivative; same atom: (eps + avg^-2)^(-1/2).
 DXTB_HD double es2_avg(int mode, double ha, double hb) {
   ha += EPS;
   hb += EPS;
@@ -56,20 +55,22 @@ DXTB_HD double es2_avg(int mode, double ha, double hb) {
   return sqrt(ha * hb);
 }
 
-DXTB_HD void es2_gamma(bool same_atom, double R, double avg, double gexp, double& g, double& dg) {
-  if (gexp == 2.0) {
-    const double ia2 = 1.0 / (avg * avg);
-    if (same_atom) {
-      g = 1.0 / sqrt(EPS + ia2);
-      dg = 0.0;
-      return;
-    }
-    const double re = R + EPS;
-    const double t = re * re + ia2;
-    g = 1.0 / sqrt(t);
-    dg = -re * g / t;
+DXTB_HD void es2_gamma(bool same_atom, double R, double avg, double& g, double& dg) {
+  const double ia2 = 1.0 / (avg * avg);
+  if (same_atom) {
+    g = 1.0 / sqrt(EPS + ia2);
+    dg = 0.0;
     return;
   }
+  const double re = R + EPS;
+  const double t = re * re + ia2;
+  g = 1.0 / sqrt(t);
+  dg = -re * g / t;
+}
+
+// The same for any exponent g > 0 (secondorder.py:799-870):
+// gamma = ((R+eps)^g + avg^-g)^(-1/g); same atom: (eps + avg^-g)^(-1/g).
+DXTB_HD void es2_gamma_gexp(bool same_atom, double R, double avg, double gexp, double& g, double& dg) {
   const double ia = pow(avg, -gexp);
   if (same_atom) {
     g = pow(EPS + ia, -1.0 / gexp);
@@ -81,6 +82,16 @@ DXTB_HD void es2_gamma(bool same_atom, double R, double avg, double gexp, double
   const double t = rg + ia;
   g = pow(t, -1.0 / gexp);
   dg = -(rg / re) * g / t;  // -R^(g-1) gamma / t
+}
+
+// Kernels are instantiated for g = 2 (es2_gamma, the code of the GFN1/GFN2
+// exponent on its own) and for any other exponent.
+template <bool G2>
+DXTB_HD void es2_gamma_t(bool same_atom, double R, double avg, double gexp, double& g, double& dg) {
+  if (G2)
+    es2_gamma(same_atom, R, avg, g, dg);
+  else
+    es2_gamma_gexp(same_atom, R, avg, gexp, g, dg);
 }
 
 // Repulsion pair energy z exp(-a R^k)/R and dE/dR (cutoff applied by caller).

@@ -63,13 +63,19 @@ int main() {
     }
     for (double gexp : {2.0, 1.7, 2.6}) {
       double g0, dg, gp, gm, t;
-      es2_gamma(false, R, 0.45, gexp, g0, dg);
-      es2_gamma(false, R + h, 0.45, gexp, gp, t);
-      es2_gamma(false, R - h, 0.45, gexp, gm, t);
+      es2_gamma_gexp(false, R, 0.45, gexp, g0, dg);
+      es2_gamma_gexp(false, R + h, 0.45, gexp, gp, t);
+      es2_gamma_gexp(false, R - h, 0.45, gexp, gm, t);
       check("gamma", dg, (gp - gm) / (2 * h), dg);
       // the closed form itself: ((R + eps)^g + avg^-g)^(-1/g)
       const double ref = std::pow(std::pow(R + EPS, gexp) + std::pow(0.45, -gexp), -1.0 / gexp);
       check("gamma value", g0, ref, ref);
+      if (gexp == 2.0) {  // the exponent-2 form against the general one
+        double g2, d2;
+        es2_gamma(false, R, 0.45, g2, d2);
+        check("gamma g=2", g2, g0, g0);
+        check("dgamma g=2", d2, dg, dg);
+      }
     }
     {
       double e, de, ep, em, t;
