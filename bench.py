@@ -209,10 +209,10 @@ def jacobi_flops_per_sweep(n: int, variant: int) -> float:
 # the upper tiles of A, hence below 1); the one-workgroup value is from 256
 # matrices before the deferred update existed.  The scalar form (1, A in LDS)
 # is not measured and counted at its algorithmic bytes.
-JACOBI_TRAFFIC_RATIO = {0: 0.968, 1: 1.0, 2: 1.154, 3: 0.968}
+JACOBI_TRAFFIC_RATIO = {0: 0.943, 1: 1.0, 2: 1.154, 3: 0.943}
 # the same with the deferred eigenvector update (256 matrices), against its
 # algorithmic bytes
-JACOBI_TRAFFIC_RATIO_DEFERRED = {0: 1.171, 1: 1.0, 2: 1.542, 3: 1.171}
+JACOBI_TRAFFIC_RATIO_DEFERRED = {0: 1.186, 1: 1.0, 2: 1.542, 3: 1.186}
 # the library's choice of the deferred eigenvector update (eigen.hip:
 # defer_call): order <= 512 and a batch capacity above 16
 DEFER_MAX, DEFER_MIN_BATCH = 512, 16
@@ -407,7 +407,7 @@ def bench_eigen(args, dev):
         "config": {"workload": f"eigensolver microbenchmark: {B} random symmetric matrices of order {n}, one sweep "
                                "per launch", "kernel_ms": t * 1e3,
                    "fp64_tflops": flops / t / 1e12, "fp64_peak_tflops": FP64_PEAK_TFLOPS},
-        "roofline": {"kernel": "k_bj_check + k_bj_inner_full + k_bj_inner_cross + k_bj_product + k_jacobi_blocked (one dxtb_jacobi_eigh call)"
+        "roofline": {"kernel": "k_bj_norms + k_bj_check + k_bj_inner_full + k_bj_step + k_bj_product (+ k_bj_apply) + k_jacobi_blocked (one dxtb_jacobi_eigh call)"
                                if variant == 3 else "k_jacobi_blocked", "bound": "hbm", "achieved": nbytes / t / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": nbytes / t / 1e9 / HBM_PEAK_GBS,
                      "traffic": (JACOBI_TRAFFIC_RATIO_DEFERRED if deferred(n, B) else JACOBI_TRAFFIC_RATIO)[variant]
