@@ -1407,13 +1407,26 @@ static_assert((size_t)PROD_TILE * sizeof(double) + sizeof(CrossLds) <= PROD_LDS,
 // panel of 32 columns), the panel's rows in LDS.
 constexpr int AW = 4;  // wavefronts per apply workgroup
 
-__global__ __launch_bounds__(AW * 64) void k_bj_apply(int nmax, const JArgs* __restrict__ ja,
+__global__ __launch_bounds__(AW * 64) void k_bj_apply(int nmax, int nbcap, const JArgs* __restrict__ ja,
                                                      double* __restrict__ work, const double* __restrict__ sst,
                                                      int* __restrict__ rot) {
   extern __shared__ double lds_raw[];
-  // grid (matrix, panel): the panels of one matrix on one XCD (as
-  // k_bj_product), which reads the sweep's rotation log once into its L2
-  const int b = blockIdx.x, pc = blockIdx.y;
+  // One workgroup per (matrix, panel) on a 1D grid.  Workgroup ids go to the
+  // eight XCDs in turn (id = 8 slot + xcd); matrix b = 8 m + xcd takes slots
+  // m npm .. m npm + npm - 1 of XCD xcd, so its panels are dispatched one
+  // after another on one XCD and read the sweep's rotation log into that
+  // XCD's L2 together (with the panels of a matrix nbcap ids apart, most of
+  // the log was gone from the L2 when the next panel read it).
+  const int npm = (nmax + SB - 1) / SB, id = blockIdx.x;
+  int b, pc;
+  if ((nbcap & 7) == 0) {
+    const int slot = id >> 3, m = slot / npm;
+    b = 8 * m + (id & 7);
+    pc = slot - m * npm;
+  } else {
+    b = id / npm;
+    pc = id - b * npm;
+  }
   const JArgsC jc = jconst(ja);
   if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
   const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, rot, jc->defer);
@@ -1499,8 +1512,8 @@ hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax,
       const hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
       if (defer)
-        hipLaunchKernelGGL(k_bj_apply, dim3((unsigned)nbcap, (unsigned)npairs), dim3(AW * 64), apply_lds(nmax), st,
-                           nmax, w.ja, w.rot_q, w.sst, w.rot);
+        hipLaunchKernelGGL(k_bj_apply, dim3((unsigned)(nbcap * npairs)), dim3(AW * 64), apply_lds(nmax), st, nmax,
+                           nbcap, w.ja, w.rot_q, w.sst, w.rot);
     }
   }
   hipLaunchKernelGGL(k_jacobi_blocked, dim3((unsigned)nbcap), dim3(JBB), sizeof(BlockedLds), st, nmax, w.ja, w.rot_q,
