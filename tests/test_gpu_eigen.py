@@ -125,25 +125,30 @@ def test_split_matches_blocked(cuda, nmax, sizes):
         torch.testing.assert_close(a @ v, v * e3[b, :n], atol=16 * n * eps * scale, rtol=0)
 
 
-@pytest.mark.parametrize("B,n", [(4, 200), (20, 100), (3, 600), (1, 157)])
-def test_split_bitwise_matches_blocked(cuda, B, n):
-    """The split form (its cross sweeps by four wavefronts per block pair)
-    and the one-workgroup form (one wavefront per block pair) apply the same
+@pytest.mark.parametrize("n,sizes", [(200, [200] * 4), (100, [100] * 20), (600, [600] * 3), (157, [157]),
+                                     (200, [200, 64, 20, 157, 0, 33]), (160, [160, 64] + [150 - 5 * k for k in range(18)])])
+def test_split_bitwise_matches_blocked(cuda, n, sizes):
+    """The split form (one launch per outer step: the products of the step
+    before, then the cross sweeps by four wavefronts per block pair) and the
+    one-workgroup form (one wavefront per block pair) apply the same
     rotations with the same arithmetic: on warm-start-like matrices (sorted
     diagonal plus a 1e-3 perturbation) they agree bit for bit, also with the
-    deferred eigenvector update (20 matrices), so an SCF result does not
-    depend on how many of its sweeps ran split."""
-    print(f"case jacobi bitwise split B={B} n={n}", flush=True)
+    deferred eigenvector update (20 matrices) and with matrices of fewer
+    blocks in the call (order 64: four blocks, whose only tile holds both
+    pairs of a step; orders up to 32: one block pair), so an SCF result does
+    not depend on how many of its sweeps ran split."""
+    B = len(sizes)
+    print(f"case jacobi bitwise split nmax={n} sizes={sizes}", flush=True)
     g = torch.Generator().manual_seed(5)
     A = torch.zeros(B, n, n, dtype=torch.float64)
-    for b in range(B):
-        d = torch.sort(torch.randn(n, generator=g, dtype=torch.float64)).values
-        M = 1e-3 * torch.randn(n, n, generator=g, dtype=torch.float64)
-        A[b] = torch.diag(d) + 0.5 * (M + M.T)
+    for b, m in enumerate(sizes):
+        d = torch.sort(torch.randn(m, generator=g, dtype=torch.float64)).values
+        M = 1e-3 * torch.randn(m, m, generator=g, dtype=torch.float64)
+        A[b, :m, :m] = torch.diag(d) + 0.5 * (M + M.T)
     A = A.to(cuda)
     for split in (2, 4):
-        e2, V2, s2 = _jacobi(A, [n] * B, 2, tol=1e-12, split=split)
-        e3, V3, s3 = _jacobi(A, [n] * B, 3, tol=1e-12, split=split)
+        e2, V2, s2 = _jacobi(A, sizes, 2, tol=1e-12, split=split)
+        e3, V3, s3 = _jacobi(A, sizes, 3, tol=1e-12, split=split)
         assert torch.equal(s2, s3)
         assert torch.equal(e2, e3) and torch.equal(V2, V3)
 
