@@ -622,6 +622,15 @@ __device__ bool bj_inner_cross(WaveLds& W, const BjMat& M, int k, int s, double 
     bji[u] = Aj[min(gix, last)] * mask01(gjj < n && gix < n);
     bjj[u] = Aj[min(gjx, last)] * mask01(gjj < n && gjx < n);
   }
+  {  // a cross block at or below the bar rotates nothing (see bj_inner_cross4)
+    bool need = false;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) need |= fabs(bij[u]) > thr;
+    if (!__any(need ? 1 : 0)) {
+      wave_sync();
+      return false;
+    }
+  }
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
     const int i = 8 * g + v;
@@ -730,6 +739,12 @@ __device__ __forceinline__ double row_share4(double v, int w, bool odd) {
 // those reads from the step's writes (the diagonal tiles hold them), another
 // orders the steps.  Returns whether this wavefront applied any rotation.
 __device__ bool inner_sweep_full4(WaveLds& W, double thr, int tid) {
+  {  // no off-diagonal element above the bar: no step would rotate (see bj_inner_cross4)
+    bool need = false;
+    for (int e = tid; e < SB * SB; e += 4 * 64)  // four wavefronts
+      need |= (e >> 5) != (e & 31) && fabs(W.S[(e >> 5) * SST + (e & 31)]) > thr;
+    if (!__syncthreads_or(need ? 1 : 0)) return false;
+  }
   const int w = tid >> 6, lane = tid & 63;
   const int r = lane >> 4;
   const int kk = ((lane & 15) + r) & 15;
@@ -809,6 +824,11 @@ __device__ bool bj_inner_cross4(CrossLds& X, const BjMat& M, int k, int s, doubl
     bji = Aj[min(gix, last)] * mask01(gjj < n && gix < n);
     bjj = Aj[min(gjx, last)] * mask01(gjj < n && gjx < n);
   }
+  // An element changes only through a rotation, and a rotation needs its
+  // off-diagonal element above the bar: a pair whose cross block is at or
+  // below the bar throughout rotates nothing in any round (the rounds would
+  // leave everything as it is), so they are skipped.
+  if (!__syncthreads_or(fabs(bij) > thr ? 1 : 0)) return false;
 #pragma unroll
   for (int v = 0; v < 2; ++v) {
     const int i = 8 * g + 2 * w + v;
@@ -889,7 +909,14 @@ __device__ bool bj_inner_pair(WaveLds& W, const BjMat& M, int k, int s, double t
     W.Q[r * SST + c] = (r == c) ? 1.0 : 0.0;
   }
   wave_sync();
-  const bool any = inner_sweep<false>(W, thr, lane);  // step 0: all pairs (later steps: bj_inner_cross)
+  bool need = false;  // an off-diagonal element above the bar (else no step would rotate)
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int e = lane + 64 * u;
+    need |= (e >> 5) != (e & 31) && fabs(v[u]) > thr;
+  }
+  // step 0: all pairs (later steps: bj_inner_cross)
+  const bool any = __any(need ? 1 : 0) && inner_sweep<false>(W, thr, lane);
   if (any) {
 #pragma unroll 4
     for (int u = 0; u < 16; ++u) {
