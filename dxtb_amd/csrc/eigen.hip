@@ -803,7 +803,8 @@ constexpr int CW = 4;
 struct CrossLds {
   double cs[2][2 * BS];          // (c, s) of the 16 pairs, by round parity
   double xb[2][2][BS][BS + 1];   // bji and bjj rows for the row advance, by round parity
-  int vote[2][CW];               // per wavefront: rotated anything this round
+  double rin[2][BS][3];          // (a_pp, a_qq, a_pq) of the 16 pairs of a round, by round parity
+  int vote[2];                   // the round rotated anything
 };
 
 __device__ bool bj_inner_cross4(CrossLds& X, const BjMat& M, int k, int s, double thr, int tid) {
@@ -824,10 +825,25 @@ __device__ bool bj_inner_cross4(CrossLds& X, const BjMat& M, int k, int s, doubl
     bji = Aj[min(gix, last)] * mask01(gjj < n && gix < n);
     bjj = Aj[min(gjx, last)] * mask01(gjj < n && gjx < n);
   }
+  // The 16 rotations of a round are computed by wavefront 0 alone (lanes
+  // 0-15), from their inputs in LDS: the lanes holding them write them there
+  // beside the row advance of the round before (the diagonal tile's a_pp and
+  // a_pq, and a_qq of the next slot, which the advance moves onto the
+  // diagonal).  One wavefront spends the rotation's ~45 dependent FP64
+  // operations instead of four, with the same two barriers per round; the
+  // values, and so the results, are those of each diagonal lane computing
+  // its own rotation.
+  const bool diag = j == x;
+  if (diag) {
+    X.rin[0][x][0] = aii;
+    X.rin[0][x][1] = bjj;
+    X.rin[0][x][2] = bij;
+  }
   // An element changes only through a rotation, and a rotation needs its
   // off-diagonal element above the bar: a pair whose cross block is at or
   // below the bar throughout rotates nothing in any round (the rounds would
-  // leave everything as it is), so they are skipped.
+  // leave everything as it is), so they are skipped.  (The barrier also
+  // publishes the first round's inputs.)
   if (!__syncthreads_or(fabs(bij) > thr ? 1 : 0)) return false;
 #pragma unroll
   for (int v = 0; v < 2; ++v) {
@@ -835,21 +851,23 @@ __device__ bool bj_inner_cross4(CrossLds& X, const BjMat& M, int k, int s, doubl
     qi[v] = (i == x) ? 1.0 : 0.0;
     qj[v] = (i == BS + x) ? 1.0 : 0.0;
   }
-  const bool diag = j == x;
   const int jn = (j + 1) & (BS - 1), xn = (x + 1) & (BS - 1);
   bool any = false;
   for (int r = 0; r < BS; ++r) {
     const int par = r & 1;
-    double c = 1.0, sn = 0.0;
-    const bool rot = diag && rotation(aii, bjj, bij, thr, c, sn);
-    if (diag) {
-      X.cs[par][2 * x] = c;
-      X.cs[par][2 * x + 1] = sn;
+    if (w == 0) {
+      bool rot = false;
+      if (lane < BS) {
+        double c = 1.0, sn = 0.0;
+        rot = rotation(X.rin[par][lane][0], X.rin[par][lane][1], X.rin[par][lane][2], thr, c, sn);
+        X.cs[par][2 * lane] = c;
+        X.cs[par][2 * lane + 1] = sn;
+      }
+      const bool wrot = __any(rot ? 1 : 0);
+      if (lane == 0) X.vote[par] = wrot ? 1 : 0;
     }
-    const bool wrot = __any(rot ? 1 : 0);
-    if (lane == 0) X.vote[par][w] = wrot ? 1 : 0;
     __syncthreads();
-    const bool arot = (X.vote[par][0] | X.vote[par][1] | X.vote[par][2] | X.vote[par][3]) != 0;
+    const bool arot = X.vote[par] != 0;
     if (arot) {  // uniform over the workgroup
       any = true;
       const double c1 = X.cs[par][2 * j], s1 = X.cs[par][2 * j + 1];
@@ -865,6 +883,11 @@ __device__ bool bj_inner_cross4(CrossLds& X, const BjMat& M, int k, int s, doubl
     for (int v = 0; v < 2; ++v) qj[v] = row_next(qj[v]);
     X.xb[par][0][j][x] = bji;
     X.xb[par][1][j][x] = bjj;
+    if (diag) {  // inputs of the next round: a_pp and a_pq of slot x, a_qq of slot x - 1
+      X.rin[par ^ 1][x][0] = aii;
+      X.rin[par ^ 1][x][2] = bij;
+      X.rin[par ^ 1][(x + BS - 1) & (BS - 1)][1] = bjj;
+    }
     __syncthreads();
     bji = X.xb[par][0][jn][x];
     bjj = X.xb[par][1][jn][xn];
