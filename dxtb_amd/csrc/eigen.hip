@@ -604,7 +604,7 @@ __global__ void k_bj_args(JArgs* __restrict__ ja, JArgs v) {
 // tile (j = x) computes the rotation of pair x; the coefficients are exchanged
 // through LDS (W.S as scratch).  Same rotations, in the same order, as the
 // cyclic cross sweep of inner_sweep.
-__device__ bool bj_inner_cross(WaveLds& W, const BjMat& M, int k, int s, double thr, int lane) {
+__device__ bool bj_inner_cross(double* cs, const BjMat& M, int k, int s, double thr, int lane) {
   int bi, bj;
   block_pair(k, s, M.nb, bi, bj);
   const int x = lane & 15, g = lane >> 4;
@@ -642,7 +642,7 @@ __device__ bool bj_inner_cross(WaveLds& W, const BjMat& M, int k, int s, double 
   const bool diag = ud >= 0 && ud < 4;
   const int down = (((g + 1) & 3) << 4) | x;        // row slot 4 g + 4, same column
   const int down_nxt = (((g + 1) & 3) << 4) | ((x + 1) & 15);
-  double* cs = W.S;  // (c, s) of the 16 pairs
+  // cs: (c, s) of the 16 pairs, 32 doubles of LDS scratch
   bool any = false;
   for (int r = 0; r < BS; ++r) {
     double app = aii[0], aqq = bjj[0], apq = bij[0];
@@ -913,7 +913,7 @@ __device__ bool bj_inner_cross4(CrossLds& X, const BjMat& M, int k, int s, doubl
 // partner; the pairs across the two blocks afterwards), write the diagonal
 // tile and the rotation back.  Returns whether anything was rotated.
 __device__ bool bj_inner_pair(WaveLds& W, const BjMat& M, int k, int s, double thr, int lane) {
-  if (s > 0) return bj_inner_cross(W, M, k, s, thr, lane);
+  if (s > 0) return bj_inner_cross(W.S, M, k, s, thr, lane);
   int bi, bj;
   block_pair(k, s, M.nb, bi, bj);
   double v[16];
@@ -1228,6 +1228,13 @@ __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const JArgs* _
 // rotation bar, done flag, sweeps done.
 
 constexpr int SPLIT_MIN_N = 128;  // auto: split form from this order, for batches below half of it
+// step kernel: one-wavefront sweeps from this many block pairs per call
+// (batch capacity x pairs).  Three sweeps, warm-start-like matrices: at 512
+// and 640 pairs (64 matrices of order 240, 320) the four-wavefront sweeps
+// are 5-7 % faster, at 768 (96 of order 256) the two are even, from 960 the
+// one-wavefront sweeps win by 2-10 % (256 matrices of order 384: 17.5
+// against 19.3 ms).
+constexpr int WAVE1_MIN_PAIRS = 768;
 
 // VT = 1 and the split state cleared: grid (matrix, chunk of INIT_ROWS rows),
 // so that a lone matrix's identity is written by many workgroups at once
@@ -1398,20 +1405,32 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(5))) vo
 }
 
 // Outer step s > 0 of the split form in one launch: the product phase of
-// step s - 1 and the cross sweeps of step s.  Workgroup y < npairs(nmax)
-// forms the tile of step s - 1 that holds A(I, J) of block pair y = (I, J) of
-// step s (wavefront 0, bj_product_tile), then sweeps the pair with all four
-// wavefronts (bj_inner_cross4: the new tile, and the diagonal blocks A(I, I),
-// A(J, J), which the sweeps of step s - 1 wrote and no product changes); the
-// workgroups above take the other product items of step s - 1, PW per
-// workgroup, as k_bj_product.  No workgroup writes what another reads: a
-// pair's sweep writes its own blocks (in its own tile and in diagonal tiles
-// of step s - 1, which no product reads) and the rotations and flags of step
-// s (kept apart from those of step s - 1).  With four blocks the only tile
-// holds both pairs: the workgroup of pair 0 sweeps them one after the other.
-// A matrix whose sweep ends before step s only forms its last products.
-// Same arithmetic as the two launches it replaces, bit for bit.
-__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(5))) void k_bj_step(
+// step s - 1 and the cross sweeps of step s.  Every block pair (I, J) of step
+// s is swept by the workgroup that forms the tile of step s - 1 holding
+// A(I, J): with WAVE1 false, workgroup y < npairs(nmax) forms it (wavefront 0,
+// bj_product_tile) and sweeps pair y with all four wavefronts
+// (bj_inner_cross4); with WAVE1, wavefront w of workgroup y < ceil(npairs / 4)
+// forms the tile of pair 4 y + w and sweeps the pair alone (bj_inner_cross,
+// wavefront barriers only; a quarter of the workgroup slots per pair, for
+// batches that fill the device).  A sweep reads the new tile and the diagonal
+// blocks A(I, I), A(J, J), which the sweeps of step s - 1 wrote and no
+// product changes.  The workgroups above take the other product items of
+// step s - 1, PW per workgroup, as k_bj_product.  No workgroup writes what
+// another reads: a pair's sweep writes its own blocks (in its own tile and in
+// diagonal tiles of step s - 1, which no product reads) and the rotations and
+// flags of step s (kept apart from those of step s - 1).  With four blocks
+// the only tile holds both pairs: the workgroup (wavefront) of pair 0 sweeps
+// them one after the other.  A matrix whose sweep ends before step s only
+// forms its last products.  The two forms of the sweep, and the products,
+// are those of the two-launch step, bit for bit.
+__host__ __device__ constexpr int sweep_groups(int nmax, bool wave1) {
+  return wave1 ? ((nmax + SB - 1) / SB + PW - 1) / PW : (nmax + SB - 1) / SB;
+}
+
+// (the one-wavefront sweep holds its 32x32 subproblem and rotation in 64
+// registers per lane: four waves per SIMD, without spills)
+template <bool WAVE1>
+__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(WAVE1 ? 4 : 5))) void k_bj_step(
     int nmax, const JArgs* __restrict__ ja, double* __restrict__ work, const double* __restrict__ sst,
     int* __restrict__ rot, int s) {
   extern __shared__ double lds_raw[];  // staging tiles; a sweep's exchange area after the first
@@ -1421,32 +1440,44 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(5))) vo
   const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, rot, jc->defer);
   if (s < 1 || s > M.nb - 1) return;
   const bool sweep = s < M.nb - 1;  // step s exists for this matrix
-  const int npmax = (nmax + SB - 1) / SB;
+  const int nsg = sweep_groups(nmax, WAVE1);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int y = blockIdx.y;
-  if (y < npmax) {
-    if (!sweep || y >= M.npairs || (M.nb == 4 && y == 1)) return;
+  if (y < nsg) {
+    const int p = WAVE1 ? PW * y + wv : y;  // the pair this wavefront (workgroup) sweeps
+    // uniform over the wavefront (WAVE1) or the workgroup
+    if (!sweep || p >= M.npairs || (M.nb == 4 && p == 1)) return;
     int bi, bj;
-    block_pair(y, s, M.nb, bi, bj);
+    block_pair(p, s, M.nb, bi, bj);
     const int ka = pair_of_block(bi, s - 1, M.nb), kb = pair_of_block(bj, s - 1, M.nb);
+    const double thr = sst[4 * b + 1];
+    const int klast = M.nb == 4 ? 1 : p;
+    if (WAVE1) {
+      double* Sx = lds_raw + (size_t)wv * PROD_TILE;
+      bj_product_tile<true>(Sx, M, fstep(M, s - 1), min(ka, kb), max(ka, kb), s - 1, lane);
+      wave_sync();  // the tile's writes to A seen by the whole wavefront; Sx free
+      for (int k = p; k <= klast; ++k) {
+        const bool any = bj_inner_cross(Sx, M, k, s, thr, lane);
+        if (lane == 0) fstep(M, s)[k] = any ? 1 : 0;
+      }
+      return;
+    }
     if (wv == 0) bj_product_tile<true>(lds_raw, M, fstep(M, s - 1), min(ka, kb), max(ka, kb), s - 1, lane);
     __syncthreads();  // the tile's writes to A seen by every wavefront
     CrossLds& X = *reinterpret_cast<CrossLds*>(lds_raw + PROD_TILE);
-    const double thr = sst[4 * b + 1];
-    const int klast = M.nb == 4 ? 1 : y;
-    for (int k = y; k <= klast; ++k) {
+    for (int k = p; k <= klast; ++k) {
       const bool any = bj_inner_cross4(X, M, k, s, thr, threadIdx.x);
       if (threadIdx.x == 0) fstep(M, s)[k] = any ? 1 : 0;
       __syncthreads();  // X is reused by the next pair
     }
     return;
   }
-  const int item = (y - npmax) * PW + wv;
+  const int item = (y - nsg) * PW + wv;
   if (item >= bj_nwork(M)) return;
   if (sweep && item < M.npairs * (M.npairs - 1) / 2) {
     int k, k2;
     tile_pairs(item, M.npairs, k, k2);
-    if (tile_holds_next_pair(M.nb, s, k, k2)) return;  // formed by the workgroup of that pair
+    if (tile_holds_next_pair(M.nb, s, k, k2)) return;  // formed by the sweeper of that pair
   }
   bj_product_item<true>(lds_raw + (size_t)wv * PROD_TILE, M, fstep(M, s - 1), item, s - 1, lane);
 }
@@ -1534,6 +1565,21 @@ Layout layout(double* work, int nbcap, int nmax) {
   return w;
 }
 
+// One-wavefront sweeps in the step kernel for batches whose step workgroups
+// fill the device (environment DXTB_JACOBI_WAVE1: 0 never, 1 always).
+int wave1_env() {
+  static const int v = [] {
+    const char* e = std::getenv("DXTB_JACOBI_WAVE1");
+    return e ? std::atoi(e) : -1;
+  }();
+  return v;
+}
+bool step_wave1(int nbcap, int nmax) {
+  const int e = wave1_env();
+  if (e >= 0) return e != 0;
+  return (long)nbcap * ((nmax + SB - 1) / SB) >= WAVE1_MIN_PAIRS;
+}
+
 // The blocked kernels of one call (after k_bj_args): nsw split sweeps, then
 // the one-workgroup kernel for the remaining max_sweeps (and the eigenvalues).
 // Every argument is fixed by (work, nbcap, nmax, nsw, max_sweeps, tol).
@@ -1553,9 +1599,16 @@ hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax,
       // launch; then the products of the last step
       hipLaunchKernelGGL(k_bj_inner_full, dim3((unsigned)npairs, (unsigned)nbcap), dim3(CW * 64), 0, st, nmax, w.ja,
                          w.rot_q, w.sst, w.rot);
-      for (int s = 1; s < nb - 1; ++s)
-        hipLaunchKernelGGL(k_bj_step, dim3((unsigned)nbcap, (unsigned)(npairs + (nwork + PW - 1) / PW)),
-                           dim3(PW * 64), PROD_LDS, st, nmax, w.ja, w.rot_q, w.sst, w.rot, s);
+      const bool wave1 = step_wave1(nbcap, nmax);
+      const unsigned ny = (unsigned)(sweep_groups(nmax, wave1) + (nwork + PW - 1) / PW);
+      for (int s = 1; s < nb - 1; ++s) {
+        if (wave1)
+          hipLaunchKernelGGL(k_bj_step<true>, dim3((unsigned)nbcap, ny), dim3(PW * 64), PROD_LDS, st, nmax, w.ja,
+                             w.rot_q, w.sst, w.rot, s);
+        else
+          hipLaunchKernelGGL(k_bj_step<false>, dim3((unsigned)nbcap, ny), dim3(PW * 64), PROD_LDS, st, nmax, w.ja,
+                             w.rot_q, w.sst, w.rot, s);
+      }
       if (nwork > 0)  // one block pair with deferred VT: nothing to multiply
         hipLaunchKernelGGL(k_bj_product, dim3((unsigned)nbcap, (unsigned)((nwork + PW - 1) / PW)), dim3(PW * 64),
                            PROD_LDS, st, nmax, w.ja, w.rot_q, w.sst, w.rot, nb - 2);

@@ -323,7 +323,10 @@ size_t dxtb_jacobi_work_size(int nbatch, int nmax);
  * (nbatch) may be NULL.  On a stream other than the null stream that is not
  * being captured, the split launches are recorded once per (workspace, batch
  * capacity, nmax, split sweeps, max_sweeps, tol) and replayed as one graph
- * (environment DXTB_JACOBI_GRAPHS=0: direct launches); calls that share a
+ * (environment DXTB_JACOBI_GRAPHS=0: direct launches; DXTB_JACOBI_WAVE1=0 or
+ * 1 fixes the split step's sweep form, four or one wavefront per block pair,
+ * which the library otherwise picks by batch capacity x block pairs, same
+ * results either way); calls that share a
  * workspace must be ordered (one stream).  Replaces the per-iteration eigensolve of
  * scf/pure/ovlp_diag.py:36-62 (xitorch exacteig / torch.linalg.eigh). */
 int dxtb_jacobi_eigh(int nbatch, int nmax, const int* nsize, double* A, double* VT, double* evals, double* work,
