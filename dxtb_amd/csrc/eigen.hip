@@ -1229,12 +1229,12 @@ __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const JArgs* _
 
 constexpr int SPLIT_MIN_N = 128;  // auto: split form from this order, for batches below half of it
 // step kernel: one-wavefront sweeps from this many block pairs per call
-// (batch capacity x pairs).  Three sweeps, warm-start-like matrices: at 512
-// and 640 pairs (64 matrices of order 240, 320) the four-wavefront sweeps
-// are 5-7 % faster, at 768 (96 of order 256) the two are even, from 960 the
-// one-wavefront sweeps win by 2-10 % (256 matrices of order 384: 17.5
-// against 19.3 ms).
-constexpr int WAVE1_MIN_PAIRS = 768;
+// (batch capacity x pairs).  Three sweeps, warm-start-like matrices: up to
+// 16 matrices the four-wavefront sweeps are 4-10 % faster, at 320 pairs (64
+// matrices of order 160) 1 %; at 512 (64 of order 240) the two are even;
+// from 640 the one-wavefront sweeps win by 1-12 % (128 matrices of order
+// 256: 2.31 against 2.67 ms; 256 of order 384: 17.4 against 19.3 ms).
+constexpr int WAVE1_MIN_PAIRS = 512;
 
 // VT = 1 and the split state cleared: grid (matrix, chunk of INIT_ROWS rows),
 // so that a lone matrix's identity is written by many workgroups at once
@@ -1428,9 +1428,10 @@ __host__ __device__ constexpr int sweep_groups(int nmax, bool wave1) {
 }
 
 // (the one-wavefront sweep holds its 32x32 subproblem and rotation in 64
-// registers per lane: four waves per SIMD, without spills)
+// registers per lane: three waves per SIMD, without spills; four spilled
+// and ran 1-9 % slower)
 template <bool WAVE1>
-__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(WAVE1 ? 4 : 5))) void k_bj_step(
+__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(WAVE1 ? 3 : 5))) void k_bj_step(
     int nmax, const JArgs* __restrict__ ja, double* __restrict__ work, const double* __restrict__ sst,
     int* __restrict__ rot, int s) {
   extern __shared__ double lds_raw[];  // staging tiles; a sweep's exchange area after the first
