@@ -88,12 +88,18 @@ __device__ __forceinline__ bool rotation(double app, double aqq, double apq, dou
     sn = 0.0;
     return false;
   }
-  // t = sign(tau) / (|tau| + sqrt(1 + tau^2)) with numerator and denominator
-  // multiplied by 2|a_pq|: one square root and one division
+  // The rotation t = sign(tau) / (|tau| + sqrt(1 + tau^2)), c = 1 / sqrt(1 +
+  // t^2), s = t c by its half angle: with r = sqrt(d^2 + 4 a_pq^2), cos 2theta
+  // = |d| / r, so c^2 = h = (1 + |d| / r) / 2 and s = sign(d) a_pq / (r c).
+  // Two reciprocal square roots and no division: about 16 dependent FP64
+  // operations where the square root and division took about 45 (this chain
+  // is the critical path of every round of a subproblem sweep).
   const double d = aqq - app;
-  const double t = (d >= 0.0 ? 2.0 * apq : -2.0 * apq) / (fabs(d) + sqrt(fma(d, d, 4.0 * apq * apq)));
-  c = rsqrt(fma(t, t, 1.0));
-  sn = t * c;
+  const double ir = rsqrt(fma(d, d, 4.0 * apq * apq));
+  const double h = fma(0.5 * fabs(d), ir, 0.5);
+  const double ih = rsqrt(h);
+  c = h * ih;
+  sn = (d >= 0.0 ? apq : -apq) * ir * ih;
   return true;
 }
 
