@@ -212,7 +212,7 @@ def jacobi_flops_per_sweep(n: int, variant: int) -> float:
 JACOBI_TRAFFIC_RATIO = {0: 0.943, 1: 1.0, 2: 1.154, 3: 0.943}
 # the same with the deferred eigenvector update (256 matrices), against its
 # algorithmic bytes
-JACOBI_TRAFFIC_RATIO_DEFERRED = {0: 1.186, 1: 1.0, 2: 1.542, 3: 1.186}
+JACOBI_TRAFFIC_RATIO_DEFERRED = {0: 1.041, 1: 1.0, 2: 1.542, 3: 1.041}
 # the library's choice of the deferred eigenvector update (eigen.hip:
 # defer_call): order <= 512 and a batch capacity above 16
 DEFER_MAX, DEFER_MIN_BATCH = 512, 16
