@@ -13,8 +13,11 @@
 //
 // Work decomposition: one 256-thread workgroup per 64x64 output tile; its
 // four wavefronts own 32x32 quadrants (2x2 blocks of 16x16 matrix-core
-// accumulators).  The k dimension is staged through LDS in slabs of 16, the
-// next slab loaded into registers while the current one is multiplied.
+// accumulators).  The k dimension is staged through LDS in slabs of 32, the
+// next slab loaded into registers while the current one is multiplied (with
+// slabs of 16 a lone order-323 product spent most of its 34-45 us in the
+// barriers of 21 slabs); the output tile is staged in the same LDS after the
+// last slab.
 // Symmetric products compute only the tiles on and above the diagonal and
 // write each also transposed, staged through LDS so that both writes are
 // coalesced.  Workgroups are numbered so that the tiles of one matrix run on
@@ -29,9 +32,12 @@ namespace {
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
 constexpr int GT = 64;        // output tile
-constexpr int GK = 16;        // k slab
+constexpr int GKS = 5;        // log2 of the k slab
+constexpr int GK = 1 << GKS;  // k slab
 constexpr int GTH = 256;      // threads per workgroup
 constexpr int GLD = GT + 4;   // LDS row stride of a staged slab
+constexpr int GLU = GK * GT / GTH;  // slab elements of one operand per thread
+constexpr int GSM = 2 * GK * GLD > GT * (GT + 1) ? 2 * GK * GLD : GT * (GT + 1);  // LDS doubles
 constexpr int NXCD = 8;       // XCDs of the MI355X
 
 struct GemmArgs {
@@ -68,10 +74,11 @@ __device__ __forceinline__ void upper_tile(int t, int T, int& ty, int& tx) {
 }
 
 template <bool TA, bool TB>
-__global__ __launch_bounds__(GTH) void k_gemm(GemmArgs g) {
-  __shared__ double As[GK * GLD];  // op(A) slab, [k][m]
-  __shared__ double Bs[GK * GLD];  // op(B) slab, [k][c]
-  __shared__ double Ot[GT * (GT + 1)];
+__global__ __launch_bounds__(GTH) __attribute__((amdgpu_waves_per_eu(3))) void k_gemm(GemmArgs g) {
+  __shared__ double smem[GSM];
+  double* const As = smem;            // op(A) slab, [k][m]
+  double* const Bs = smem + GK * GLD;  // op(B) slab, [k][c]
+  double* const Ot = smem;            // the output tile, after the last slab
 
   const int T = g.tiles;
   const int per = g.sym ? T * (T + 1) / 2 : T * T;
@@ -118,20 +125,20 @@ __global__ __launch_bounds__(GTH) void k_gemm(GemmArgs g) {
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int last = nb - 1;
 
-  double ra[4], rb[4];
+  double ra[GLU], rb[GLU];
   // op(A)[m][k] and op(B)[k][c] of slab k0 into registers: masked loads from
   // clamped (valid) addresses, all in flight together
   auto load = [&](int k0) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < GLU; ++u) {
       const int idx = tid + GTH * u;
       int m, k;
       if (TA) {  // A stored [k][m]: 64 contiguous m per k
         k = idx >> 6;
         m = idx & 63;
-      } else {  // A stored [m][k]: 16 contiguous k per m
-        m = idx >> 4;
-        k = idx & 15;
+      } else {  // A stored [m][k]: GK contiguous k per m
+        m = idx >> GKS;
+        k = idx & (GK - 1);
       }
       const int row = r0 + m, kk = k0 + k;
       const int rr = min(row, last), kc = min(kk, last);
@@ -139,8 +146,8 @@ __global__ __launch_bounds__(GTH) void k_gemm(GemmArgs g) {
       ra[u] = x * mask01(row < nb && kk < nb);
       int c, k2;
       if (TB) {  // B stored [c][k]
-        c = idx >> 4;
-        k2 = idx & 15;
+        c = idx >> GKS;
+        k2 = idx & (GK - 1);
       } else {  // B stored [k][c]
         k2 = idx >> 6;
         c = idx & 63;
@@ -154,11 +161,11 @@ __global__ __launch_bounds__(GTH) void k_gemm(GemmArgs g) {
   };
   auto store = [&]() {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < GLU; ++u) {
       const int idx = tid + GTH * u;
-      const int m = TA ? (idx & 63) : (idx >> 4), k = TA ? (idx >> 6) : (idx & 15);
+      const int m = TA ? (idx & 63) : (idx >> GKS), k = TA ? (idx >> 6) : (idx & (GK - 1));
       As[k * GLD + m] = ra[u];
-      const int c = TB ? (idx >> 4) : (idx & 63), k2 = TB ? (idx & 15) : (idx >> 6);
+      const int c = TB ? (idx >> GKS) : (idx & 63), k2 = TB ? (idx & (GK - 1)) : (idx >> 6);
       Bs[k2 * GLD + c] = rb[u];
     }
   };
@@ -189,7 +196,9 @@ __global__ __launch_bounds__(GTH) void k_gemm(GemmArgs g) {
     }
   }
 
-  // tile -> LDS, then coalesced writes (and the transposed copy)
+  // tile -> LDS (over the slabs: every wavefront has read the last one), then
+  // coalesced writes (and the transposed copy)
+  __syncthreads();
 #pragma unroll
   for (int rb_ = 0; rb_ < 2; ++rb_)
 #pragma unroll
