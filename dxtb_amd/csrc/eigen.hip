@@ -1493,7 +1493,10 @@ static_assert((size_t)PROD_TILE * sizeof(double) + sizeof(CrossLds) <= PROD_LDS,
 
 // Deferred eigenvector update of one sweep: one workgroup per (matrix, VT
 // panel of 32 columns), the panel's rows in LDS.
-constexpr int AW = 4;  // wavefronts per apply workgroup
+// wavefronts per apply workgroup: the panel's LDS (all rows x 32 columns) lets
+// one or two workgroups share a CU, so eight wavefronts take a step's block
+// pairs one each (four took two each, one after the other)
+constexpr int AW = 8;
 
 __global__ __launch_bounds__(AW * 64) void k_bj_apply(int nmax, int nbcap, const JArgs* __restrict__ ja,
                                                      double* __restrict__ work, const double* __restrict__ sst,
