@@ -126,7 +126,8 @@ def test_split_matches_blocked(cuda, nmax, sizes):
 
 
 @pytest.mark.parametrize("n,sizes", [(200, [200] * 4), (100, [100] * 20), (600, [600] * 3), (157, [157]),
-                                     (200, [200, 64, 20, 157, 0, 33]), (160, [160, 64] + [150 - 5 * k for k in range(18)])])
+                                     (200, [200, 64, 20, 157, 0, 33]), (160, [160, 64] + [150 - 5 * k for k in range(18)]),
+                                     (224, [224, 64, 20, 0] + [210 - k for k in range(76)])])
 def test_split_bitwise_matches_blocked(cuda, n, sizes):
     """The split form (one launch per outer step: the products of the step
     before, then the cross sweeps by four wavefronts per block pair) and the
@@ -135,8 +136,10 @@ def test_split_bitwise_matches_blocked(cuda, n, sizes):
     diagonal plus a 1e-3 perturbation) they agree bit for bit, also with the
     deferred eigenvector update (20 matrices) and with matrices of fewer
     blocks in the call (order 64: four blocks, whose only tile holds both
-    pairs of a step; orders up to 32: one block pair), so an SCF result does
-    not depend on how many of its sweeps ran split."""
+    pairs of a step; orders up to 32: one block pair) and in calls of 512
+    block pairs or more (80 matrices of order up to 224: 96 x 7 pairs), whose
+    steps sweep every pair with one wavefront, so an SCF result does not
+    depend on how many of its sweeps ran split."""
     B = len(sizes)
     print(f"case jacobi bitwise split nmax={n} sizes={sizes}", flush=True)
     g = torch.Generator().manual_seed(5)
