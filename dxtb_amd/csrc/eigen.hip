@@ -798,19 +798,18 @@ __device__ bool inner_sweep_full4(WaveLds& W, double thr, int tid) {
 // form, outer steps s > 0).  Same slot space, rotations and update formulas;
 // lane l of wavefront w owns the single tile at row slot j = 4 (l / 16) + w,
 // column slot x = l % 16 of the four 16x16 blocks, and the Q rows
-// i = 8 (l / 16) + 2 w + v (v < 2).  Per round: the 16 lanes holding a
-// diagonal tile (j = x) compute the rotations, which all wavefronts read from
-// LDS after a barrier; the J-side column advance stays within each row of 16
-// lanes (DPP), the J-side row advance (slot j takes j + 1, which lives in the
-// next wavefront) goes through LDS.  Both LDS exchanges are double buffered by
-// round parity, so a round needs two barriers.
+// i = 8 (l / 16) + 2 w + v (v < 2).  Per round: every lane computes the
+// rotation of its column slot x from the round's inputs in LDS and takes
+// that of its row slot j from lane j of its wavefront; the J-side column
+// advance stays within each row of 16 lanes (DPP), the J-side row advance
+// (slot j takes j + 1, which lives in the next wavefront) goes through LDS.
+// The LDS exchanges are double buffered by round parity, so a round needs
+// one barrier.
 constexpr int CW = 4;
 
 struct CrossLds {
-  double cs[2][2 * BS];          // (c, s) of the 16 pairs, by round parity
   double xb[2][2][BS][BS + 1];   // bji and bjj rows for the row advance, by round parity
   double rin[2][BS][3];          // (a_pp, a_qq, a_pq) of the 16 pairs of a round, by round parity
-  int vote[2];                   // the round rotated anything
 };
 
 __device__ bool bj_inner_cross4(CrossLds& X, const BjMat& M, int k, int s, double thr, int tid) {
@@ -831,14 +830,13 @@ __device__ bool bj_inner_cross4(CrossLds& X, const BjMat& M, int k, int s, doubl
     bji = Aj[min(gix, last)] * mask01(gjj < n && gix < n);
     bjj = Aj[min(gjx, last)] * mask01(gjj < n && gjx < n);
   }
-  // The 16 rotations of a round are computed by wavefront 0 alone (lanes
-  // 0-15), from their inputs in LDS: the lanes holding them write them there
-  // beside the row advance of the round before (the diagonal tile's a_pp and
-  // a_pq, and a_qq of the next slot, which the advance moves onto the
-  // diagonal).  One wavefront spends the rotation's ~45 dependent FP64
-  // operations instead of four, with the same two barriers per round; the
-  // values, and so the results, are those of each diagonal lane computing
-  // its own rotation.
+  // The 16 rotations of a round are computed from their inputs in LDS, which
+  // the lanes holding them write there beside the row advance of the round
+  // before (the diagonal tile's a_pp and a_pq, and a_qq of the next slot,
+  // which the advance moves onto the diagonal).  Every wavefront computes all
+  // 16 (each lane that of its column slot), so no second barrier hands them
+  // out; the values, and so the results, are those of each diagonal lane
+  // computing its own rotation.
   const bool diag = j == x;
   if (diag) {
     X.rin[0][x][0] = aii;
@@ -861,23 +859,12 @@ __device__ bool bj_inner_cross4(CrossLds& X, const BjMat& M, int k, int s, doubl
   bool any = false;
   for (int r = 0; r < BS; ++r) {
     const int par = r & 1;
-    if (w == 0) {
-      bool rot = false;
-      if (lane < BS) {
-        double c = 1.0, sn = 0.0;
-        rot = rotation(X.rin[par][lane][0], X.rin[par][lane][1], X.rin[par][lane][2], thr, c, sn);
-        X.cs[par][2 * lane] = c;
-        X.cs[par][2 * lane + 1] = sn;
-      }
-      const bool wrot = __any(rot ? 1 : 0);
-      if (lane == 0) X.vote[par] = wrot ? 1 : 0;
-    }
-    __syncthreads();
-    const bool arot = X.vote[par] != 0;
-    if (arot) {  // uniform over the workgroup
+    double c2, s2;
+    const bool rot = rotation(X.rin[par][x][0], X.rin[par][x][1], X.rin[par][x][2], thr, c2, s2);
+    // every wavefront holds all 16 column slots: the vote is the same in each
+    if (__any(rot ? 1 : 0)) {  // uniform over the workgroup
       any = true;
-      const double c1 = X.cs[par][2 * j], s1 = X.cs[par][2 * j + 1];
-      const double c2 = X.cs[par][2 * x], s2 = X.cs[par][2 * x + 1];
+      const double c1 = __shfl(c2, j), s1 = __shfl(s2, j);  // lane j holds column slot j
       tile_rot(c1, s1, c2, s2, aii, bij, bji, bjj);  // tile (j, x) <- R_j^T tile R_x
 #pragma unroll
       for (int v = 0; v < 2; ++v) qrow_rot(c2, s2, qi[v], qj[v]);  // Q <- Q J: columns I_x and J_sigma(x)
