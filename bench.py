@@ -123,7 +123,7 @@ def bench_integrals(args, dev):
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
     fwd = float(np.mean(fwd_ms))
-    # fwd includes the output zero-fill (torch.zeros of S, D, Q)
+    # fwd includes zeroing the outputs' padding (dxtb_zero_padding of S, D, Q)
     achieved = out_bytes / (fwd * 1e-3) / 1e9
     res = {
         "metric": "integral-driver shell pairs/s (S+D+Q forward + d/dR VJP)",
@@ -136,7 +136,7 @@ def bench_integrals(args, dev):
                    "ao_order": ("shells grouped by angular momentum (the serving pools' layout)" if args.ao_order == "l"
                                 else "reference (atoms in order; the integral driver's layout)")},
         "roofline": {
-            "kernel": "multipole_fwd (9 class launches + output zero fill)",
+            "kernel": "multipole_fwd (9 class launches + padding zero fill)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -83,6 +83,7 @@ class DxtbGbParams(ctypes.Structure):
 
 _SIGNATURES: dict[str, list] = {
     "dxtb_multipole_fwd": [POINTER(DxtbBasis)] + [c_void_p] * 6 + [c_int, c_void_p],
+    "dxtb_zero_padding": [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p],
     "dxtb_multipole_vjp": [POINTER(DxtbBasis), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "dxtb_pair_bounds": [POINTER(DxtbBasis), c_void_p, c_void_p, c_void_p, c_void_p],
     "dxtb_setup": [POINTER(DxtbBasis), POINTER(DxtbParams)] + [c_void_p] * 9 + [c_void_p],

@@ -148,6 +148,9 @@ class BatchBasis:
     # shells in order) or "l" (the shells grouped by angular momentum: every
     # s shell, then every p, then every d, each group in reference order)
     ao_order: str = "reference"
+    # the work lists hold every shell pair of every molecule (build_batch),
+    # so the integral kernels write each molecule's whole leading block
+    complete_pairs: bool = False
 
     @property
     def nbatch(self) -> int:
@@ -301,5 +304,5 @@ def build_batch(numbers, gfn: str, device: torch.device | str, dims: tuple[int, 
     }
     return BatchBasis(
         gfn, nums, nat, nsh, nao, int(nat_max), nsh_max, nao_max, sh_atom, sh_ush, sh_ao, sh_l,
-        ao_atom, ao_shell, item_mol, item_pair, class_off, d, ao_order,
+        ao_atom, ao_shell, item_mol, item_pair, class_off, d, ao_order, complete_pairs=True,
     )

@@ -644,7 +644,37 @@ bool basis_ok(const dxtb_basis_t* b) {
   return true;
 }
 
+// Padding of nmat stacked (n, n) matrices per molecule: rows and columns at
+// or beyond nao[b].  One workgroup per (matrix, 16 rows), a wavefront per
+// row in turn, lanes along the row (coalesced writes); a row inside the
+// molecule has only its columns from nao[b] on written.
+constexpr int ZP_ROWS = 16;
+
+__global__ __launch_bounds__(256) void k_zero_padding(int n, int nmat, int rblocks, const int* __restrict__ nao,
+                                                      double* __restrict__ X) {
+  const long id = blockIdx.x;
+  const long mat = id / rblocks;  // b * nmat + component
+  const int rb = (int)(id - mat * rblocks);
+  const int m = min(max(nao[mat / nmat], 0), n);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int r = rb * ZP_ROWS + w; r < min(n, (rb + 1) * ZP_ROWS); r += 4) {
+    double* row = X + ((size_t)mat * n + r) * n;
+    for (int c = (r < m ? m : 0) + lane; c < n; c += 64) row[c] = 0.0;
+  }
+}
+
 }  // namespace
+
+extern "C" int dxtb_zero_padding(int nbatch, int n, int nmat, const int* nao, double* X, hipStream_t stream) {
+  if (nbatch < 0 || n < 0 || nmat <= 0 || (nbatch > 0 && n > 0 && (nao == nullptr || X == nullptr)))
+    return DXTB_EINVAL;
+  if (nbatch == 0 || n == 0) return 0;
+  const int rblocks = (n + ZP_ROWS - 1) / ZP_ROWS;
+  const long nblocks = (long)nbatch * nmat * rblocks;
+  if (nblocks > 0x7fffffffL) return DXTB_EINVAL;
+  hipLaunchKernelGGL(k_zero_padding, dim3((unsigned)nblocks), dim3(256), 0, stream, n, nmat, rblocks, nao, X);
+  return (int)hipGetLastError();
+}
 
 extern "C" int dxtb_multipole_fwd(const dxtb_basis_t* bas, const double* pos, const double* fsh, double* S,
                                   double* D, double* Q, double* H0, int traceless, hipStream_t stream) {

@@ -58,14 +58,23 @@ def multipole_forward(bb: BatchBasis, positions: torch.Tensor, multipoles: bool 
     dev = pos.device
     n = bb.nao_max
     B = bb.nbatch
-    S = torch.zeros((B, n, n), dtype=torch.float64, device=dev)
-    D = torch.zeros((B, 3, n, n), dtype=torch.float64, device=dev) if multipoles else None
-    Q = torch.zeros((B, 6, n, n), dtype=torch.float64, device=dev) if multipoles else None
-    H0 = torch.zeros((B, n, n), dtype=torch.float64, device=dev) if fsh is not None else None
+    # complete work lists write every element of each molecule's block: only
+    # the padding is zeroed (screened lists leave blocks unwritten)
+    alloc = torch.empty if bb.complete_pairs else torch.zeros
+    S = alloc((B, n, n), dtype=torch.float64, device=dev)
+    D = alloc((B, 3, n, n), dtype=torch.float64, device=dev) if multipoles else None
+    Q = alloc((B, 6, n, n), dtype=torch.float64, device=dev) if multipoles else None
+    H0 = alloc((B, n, n), dtype=torch.float64, device=dev) if fsh is not None else None
+    stream = _lib.stream_ptr(dev)
+    if bb.complete_pairs:
+        for X, nmat in ((S, 1), (D, 3), (Q, 6), (H0, 1)):
+            if X is not None:
+                _lib.check(lib.dxtb_zero_padding(B, n, nmat, _lib.ptr(bb.dev["nao"]), _lib.ptr(X), stream),
+                           "dxtb_zero_padding")
     st = _lib.basis_struct(bb)
     rc = lib.dxtb_multipole_fwd(
         st, _lib.ptr(pos), _lib.ptr(fsh), _lib.ptr(S), _lib.ptr(D), _lib.ptr(Q), _lib.ptr(H0),
-        1 if traceless else 0, _lib.stream_ptr(dev),
+        1 if traceless else 0, stream,
     )
     _lib.check(rc, "dxtb_multipole_fwd")
     return S, D, Q, H0

@@ -62,6 +62,15 @@ typedef struct dxtb_basis {
 int dxtb_multipole_fwd(const dxtb_basis_t* bas, const double* pos, const double* fsh,
                        double* S, double* D, double* Q, double* H0, int traceless, hipStream_t stream);
 
+/* Zeroes the padding of X (nbatch, nmat, n, n): in each matrix of molecule b
+ * the rows and columns at or beyond nao[b] (nao: (nbatch) device ints).  With
+ * the complete shell-pair work lists of a batch, dxtb_multipole_fwd writes
+ * every element of each molecule's leading (nao[b], nao[b]) block, so this
+ * replaces zeroing the whole output beforehand (the reference allocates its
+ * matrices as zeros, impls/pairs.py:218); screened work lists leave blocks
+ * unwritten and need the zeroed output. */
+int dxtb_zero_padding(int nbatch, int n, int nmat, const int* nao, double* X, hipStream_t stream);
+
 /* Nuclear gradient of sum(adjS*S) + sum(adjD*D) + sum(adjQ*Q) (the VJP of
  * dxtb_multipole_fwd).  adjD/adjQ may be NULL.  grad (nbatch, nat_max, 3) is
  * accumulated into (caller zeroes it).  Replaces autograd through the

@@ -205,3 +205,60 @@ def test_integrals_container_and_wrappers(cuda, geometries):
     torch.testing.assert_close(overlap(nums, pos, "gfn2").cpu(), s, atol=TOL, rtol=0)
     torch.testing.assert_close(dipint(nums, pos, "gfn2").cpu(), d, atol=TOL, rtol=0)
     torch.testing.assert_close(quadint(nums, pos, "gfn2").cpu(), q, atol=TOL, rtol=0)
+
+
+@pytest.mark.parametrize("order", ["reference", "l"])
+@pytest.mark.parametrize("gfn", ["gfn1", "gfn2"])
+def test_complete_lists_write_every_element(cuda, gfn, order):
+    """With complete work lists the forward kernels write every element of
+    each molecule's leading block: outputs filled with NaN beforehand, their
+    padding zeroed by dxtb_zero_padding, end up bit for bit equal to outputs
+    zeroed whole (the path multipole_forward takes for screened lists)."""
+    from dxtb_amd import _lib
+
+    lib = _lib.load()
+    nums, poss, _ = workloads.config_batch(1, count=3)
+    n2, p2, _ = workloads.config_batch(2, count=3)
+    mols = list(zip(nums, poss)) + [(n2[0], p2[0]), (n2[2], p2[2])]
+    print(f"case complete lists {gfn} {order}", flush=True)
+    N, P = workloads.pad([m[0] for m in mols], [m[1] for m in mols])
+    pos = torch.as_tensor(P, dtype=torch.float64, device=cuda)
+    bb = hbasis.build_batch(N, gfn, cuda, ao_order=order)
+    assert bb.complete_pairs and len(set(bb.nao.tolist())) > 1
+    B, n, ns = bb.nbatch, bb.nao_max, bb.nsh_max
+    g = torch.Generator().manual_seed(3)
+    fsh = torch.randn(B, ns, ns, generator=g, dtype=torch.float64).to(cuda)
+    st = _lib.basis_struct(bb)
+    stream = _lib.stream_ptr(cuda)
+    shapes = [(B, n, n), (B, 3, n, n), (B, 6, n, n), (B, n, n)]
+    for traceless in (1, 0):
+        ref = [torch.zeros(s, dtype=torch.float64, device=cuda) for s in shapes]
+        out = [torch.full(s, float("nan"), dtype=torch.float64, device=cuda) for s in shapes]
+        for X, nmat in zip(out, (1, 3, 6, 1)):
+            _lib.check(lib.dxtb_zero_padding(B, n, nmat, _lib.ptr(bb.dev["nao"]), _lib.ptr(X), stream),
+                       "dxtb_zero_padding")
+        for M in (ref, out):
+            _lib.check(lib.dxtb_multipole_fwd(st, _lib.ptr(pos), _lib.ptr(fsh), *[_lib.ptr(X) for X in M],
+                                              traceless, stream), "dxtb_multipole_fwd")
+        torch.cuda.synchronize(cuda)
+        for a, b in zip(out, ref):
+            assert not torch.isnan(a).any()
+            assert torch.equal(a, b)
+    # multipole_forward itself (empty outputs, padding zeroed) against the zeroed path
+    S, D, Q, H0 = hint.multipole_forward(bb, pos, fsh=fsh)
+    zs = [torch.zeros(s, dtype=torch.float64, device=cuda) for s in shapes]
+    _lib.check(lib.dxtb_multipole_fwd(st, _lib.ptr(pos), _lib.ptr(fsh), *[_lib.ptr(X) for X in zs], 1, stream),
+               "dxtb_multipole_fwd")
+    torch.cuda.synchronize(cuda)
+    for a, b in zip((S, D, Q, H0), zs):
+        assert torch.equal(a, b)
+
+
+def test_zero_padding_rejects_bad_arguments():
+    from dxtb_amd import _lib
+
+    lib = _lib.load()
+    assert lib.dxtb_zero_padding(1, 4, 0, None, None, None) != 0
+    assert lib.dxtb_zero_padding(-1, 4, 1, None, None, None) != 0
+    assert lib.dxtb_zero_padding(2, 4, 1, None, None, None) != 0
+    assert lib.dxtb_zero_padding(0, 4, 1, None, None, None) == 0
