@@ -645,19 +645,17 @@ bool basis_ok(const dxtb_basis_t* b) {
 }
 
 // Padding of nmat stacked (n, n) matrices per molecule: rows and columns at
-// or beyond nao[b].  One workgroup per (matrix, 16 rows), a wavefront per
-// row in turn, lanes along the row (coalesced writes); a row inside the
-// molecule has only its columns from nao[b] on written.
-constexpr int ZP_ROWS = 16;
-
-__global__ __launch_bounds__(256) void k_zero_padding(int n, int nmat, int rblocks, const int* __restrict__ nao,
+// or beyond nao[b].  One workgroup per matrix (with a workgroup per 16 rows
+// the dispatch of the many small workgroups took longer than the writes),
+// its four wavefronts taking the rows in turn, lanes along the row
+// (coalesced writes); a row inside the molecule has only its columns from
+// nao[b] on written.
+__global__ __launch_bounds__(256) void k_zero_padding(int n, int nmat, const int* __restrict__ nao,
                                                       double* __restrict__ X) {
-  const long id = blockIdx.x;
-  const long mat = id / rblocks;  // b * nmat + component
-  const int rb = (int)(id - mat * rblocks);
+  const long mat = blockIdx.x;  // b * nmat + component
   const int m = min(max(nao[mat / nmat], 0), n);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int r = rb * ZP_ROWS + w; r < min(n, (rb + 1) * ZP_ROWS); r += 4) {
+  for (int r = w; r < n; r += 4) {
     double* row = X + ((size_t)mat * n + r) * n;
     for (int c = (r < m ? m : 0) + lane; c < n; c += 64) row[c] = 0.0;
   }
@@ -669,10 +667,9 @@ extern "C" int dxtb_zero_padding(int nbatch, int n, int nmat, const int* nao, do
   if (nbatch < 0 || n < 0 || nmat <= 0 || (nbatch > 0 && n > 0 && (nao == nullptr || X == nullptr)))
     return DXTB_EINVAL;
   if (nbatch == 0 || n == 0) return 0;
-  const int rblocks = (n + ZP_ROWS - 1) / ZP_ROWS;
-  const long nblocks = (long)nbatch * nmat * rblocks;
+  const long nblocks = (long)nbatch * nmat;
   if (nblocks > 0x7fffffffL) return DXTB_EINVAL;
-  hipLaunchKernelGGL(k_zero_padding, dim3((unsigned)nblocks), dim3(256), 0, stream, n, nmat, rblocks, nao, X);
+  hipLaunchKernelGGL(k_zero_padding, dim3((unsigned)nblocks), dim3(256), 0, stream, n, nmat, nao, X);
   return (int)hipGetLastError();
 }
 
