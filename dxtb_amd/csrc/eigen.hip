@@ -1484,8 +1484,16 @@ static_assert((size_t)PROD_TILE * sizeof(double) + sizeof(CrossLds) <= PROD_LDS,
 // one or two workgroups share a CU, so eight wavefronts take a step's block
 // pairs one each (four took two each, one after the other)
 constexpr int AW = 8;
+// Above eight block pairs (orders 257-512) a step's pairs would take two
+// passes of eight wavefronts, the second one partly idle: the workgroup has
+// a wavefront per block pair instead (the panel's operations, and so its
+// values, do not depend on which wavefront applies a pair).
+constexpr int AW_MAX = 16;
+__host__ __device__ constexpr int apply_waves(int npairs) {
+  return npairs > AW ? (npairs < AW_MAX ? npairs : AW_MAX) : AW;
+}
 
-__global__ __launch_bounds__(AW * 64) void k_bj_apply(int nmax, int nbcap, const JArgs* __restrict__ ja,
+__global__ __launch_bounds__(AW_MAX * 64) void k_bj_apply(int nmax, int nbcap, const JArgs* __restrict__ ja,
                                                      double* __restrict__ work, const double* __restrict__ sst,
                                                      int* __restrict__ rot) {
   extern __shared__ double lds_raw[];
@@ -1509,7 +1517,7 @@ __global__ __launch_bounds__(AW * 64) void k_bj_apply(int nmax, int nbcap, const
   if (b >= jc->nbatch || sst[4 * b + 2] != 0.0) return;
   const BjMat M = bj_mat(b, nmax, jc->nsize, jc->A, jc->VT, work, rot, jc->defer);
   if (pc >= M.npairs) return;  // n <= 32 npairs: one panel per block pair
-  bj_apply_panel(M, pc, lds_raw, threadIdx.x >> 6, AW, threadIdx.x & 63);
+  bj_apply_panel(M, pc, lds_raw, threadIdx.x >> 6, (int)(blockDim.x >> 6), threadIdx.x & 63);
 }
 
 bool attr_done[64];
@@ -1612,7 +1620,7 @@ hipError_t enqueue_blocked(hipStream_t st, const Layout& w, int nbcap, int nmax,
       const hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
       if (defer)
-        hipLaunchKernelGGL(k_bj_apply, dim3((unsigned)(nbcap * npairs)), dim3(AW * 64), apply_lds(nmax), st, nmax,
+        hipLaunchKernelGGL(k_bj_apply, dim3((unsigned)(nbcap * npairs)), dim3(apply_waves(npairs) * 64), apply_lds(nmax), st, nmax,
                            nbcap, w.ja, w.rot_q, w.sst, w.rot);
     }
   }
