@@ -1222,12 +1222,13 @@ __global__ __launch_bounds__(JBB) void k_jacobi_blocked(int nmax, const JArgs* _
 
 constexpr int SPLIT_MIN_N = 128;  // auto: split form from this order, for batches below half of it
 // step kernel: one-wavefront sweeps from this many block pairs per call
-// (batch capacity x pairs).  Three sweeps, warm-start-like matrices: up to
-// 16 matrices the four-wavefront sweeps are 4-10 % faster, at 320 pairs (64
-// matrices of order 160) 1 %; at 512 (64 of order 240) the two are even;
-// from 640 the one-wavefront sweeps win by 1-12 % (128 matrices of order
-// 256: 2.31 against 2.67 ms; 256 of order 384: 17.4 against 19.3 ms).
-constexpr int WAVE1_MIN_PAIRS = 512;
+// (batch capacity x pairs).  Three sweeps, warm-start-like matrices, with
+// one barrier per round of the four-wavefront sweep: up to 16 matrices the
+// four-wavefront sweeps are 13-17 % faster, at 320 pairs (64 matrices of
+// order 160) 5 %, at 512 (64 of order 240) 3 %; from 640 (64 of order 320)
+// the one-wavefront sweeps win by 5-14 % (128 matrices of order 256: 2.27
+// against 2.59 ms; 256 of order 384: 15.1 against 16.8 ms).
+constexpr int WAVE1_MIN_PAIRS = 576;
 
 // VT = 1 and the split state cleared: grid (matrix, chunk of INIT_ROWS rows),
 // so that a lone matrix's identity is written by many workgroups at once

@@ -136,7 +136,7 @@ def test_split_bitwise_matches_blocked(cuda, n, sizes):
     diagonal plus a 1e-3 perturbation) they agree bit for bit, also with the
     deferred eigenvector update (20 matrices) and with matrices of fewer
     blocks in the call (order 64: four blocks, whose only tile holds both
-    pairs of a step; orders up to 32: one block pair) and in calls of 512
+    pairs of a step; orders up to 32: one block pair) and in calls of 576
     block pairs or more (80 matrices of order up to 224: 96 x 7 pairs), whose
     steps sweep every pair with one wavefront, so an SCF result does not
     depend on how many of its sweeps ran split."""
